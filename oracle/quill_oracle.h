@@ -1,0 +1,103 @@
+/*
+ * oracle/quill_oracle.h -- FP64 CPU restatement of QUILL's per-step PIC loop
+ * on one subdomain (n_sr = 1).
+ *
+ * TEST INFRASTRUCTURE.  This is the checker the GPU path is compared with; it
+ * is never linked into, loaded by or called from the product (quill_amd/).
+ * Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * use it.
+ *
+ * Layout follows the reference containers (include/containers.h:17-52):
+ * every field is an AoS array double[nx][ny][nz][3], k fastest.  Positions
+ * are absolute, in cell units from the subdomain origin (src/main.cpp:2573,
+ * src/advance.cpp:281-285).
+ */
+#ifndef QUILL_ORACLE_H
+#define QUILL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OQ_FDTD = 0, OQ_NDFX = 1, OQ_FIVE_POINT = 2 };   /* containers.h:82-84 */
+enum { OQ_VAY = 0, OQ_BORIS = 1 };                      /* containers.h:86-88 */
+enum { OQ_E = 0, OQ_B = 1, OQ_J = 2, OQ_BE = 3 };
+
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz, dt;        /* c/omega units (config value * 2 pi) */
+    int solver;                   /* OQ_FDTD / OQ_NDFX / OQ_FIVE_POINT */
+    int pusher;                   /* OQ_VAY / OQ_BORIS */
+    int qed;                      /* photon emission + Breit-Wheeler + vacuum pairs */
+    double e_s;                   /* Sauter field, mc omega / e units */
+    double external_bz;           /* added to the gathered Bz (advance.cpp:136) */
+    double q_vac;                 /* charge of vacuum-born macro-particles */
+    uint64_t seed;                /* QED random stream */
+    double norm_n;                /* compute_N multiplier (fout.cpp:82-84) */
+    double norm_ef;               /* compute_energy field multiplier (fout.cpp:132) */
+    double norm_ep;               /* compute_energy particle multiplier (fout.cpp:133-137) */
+} oq_params;
+
+typedef struct {
+    double N_e, N_p, N_ph;                 /* fout.cpp:41-85 */
+    int64_t n_e, n_p, n_ph, n_ion;         /* macro-particle counts */
+    double energy_f, energy_e, energy_p, energy_ph, energy_ion;  /* fout.cpp:87-138 */
+} oq_diag;
+
+typedef struct oq_state oq_state;
+
+oq_state *oq_create(const oq_params *p);
+void oq_destroy(oq_state *s);
+/* pointer to one field array double[nx][ny][nz][3] (OQ_E, OQ_B, OQ_J, OQ_BE) */
+double *oq_field(oq_state *s, int which);
+/* particles: absolute positions (cell units) and u = p/mc; g is computed as in
+ * fill_cell_by_particles (init.cpp:461-464); place() semantics drop those
+ * outside is_inside (plist.cpp:42-48).  Returns number kept. */
+int64_t oq_add_particles(oq_state *s, int64_t n, const double *x, const double *y, const double *z,
+                         const double *ux, const double *uy, const double *uz, const double *q,
+                         double cmr, const int32_t *id);
+int64_t oq_count(const oq_state *s);
+/* copy out: x,y,z,ux,uy,uz,g,q,cmr,chi (10 doubles per particle, row-major) and ids */
+void oq_get_particles(const oq_state *s, double *rec10, int32_t *id);
+
+void oq_init_boundaries(oq_state *s);     /* solver->init_boundaries(); interpolate_be() */
+void oq_interpolate_be(oq_state *s);      /* advance.cpp:20-83 */
+void oq_maxwell(oq_state *s);             /* maxwell.cpp:10-22 */
+void oq_padvance(oq_state *s);            /* advance.cpp:90-387 */
+void oq_birth_from_vacuum(oq_state *s);   /* mc.cpp:142-196 */
+void oq_diagnostics(oq_state *s, oq_diag *d);   /* fout.cpp:41-138 */
+void oq_compute_rho(oq_state *s);         /* fout.cpp:7-39 (rho stored in J slots) */
+/* one full step in the order of src/main.cpp:2319-2332; diag may be NULL */
+void oq_step(oq_state *s, int nsteps, oq_diag *diag);
+
+/* laser initial condition, f_init_cos with sscos = 0 (init.cpp:6-210) */
+void oq_init_laser_cos(oq_state *s, double a0y, double a0z, double xsigma, double ysigma,
+                       double zsigma, double x0, int b_sign, double x1, double phase,
+                       double y0, double z0, int append, double phi,
+                       double xtarget, double ytarget, double ztarget);
+
+/* uniform thermal electron plasma from the shared Philox loader (philox.h);
+ * cells with i in [ilo, ihi) (and is_inside); returns particles added */
+int64_t oq_load_uniform(oq_state *s, int ppc, double density_ncr, double sigma_u,
+                        uint64_t seed, int ilo, int ihi);
+
+/* the same loader writing to caller arrays: rec8 holds x,y,z,ux,uy,uz,q,cmr
+ * per particle (row-major), id the global particle id.  Pass rec8 = NULL to
+ * count only.  Returns the number of particles. */
+int64_t oq_synth_plasma(int nx, int ny, int nz, int ppc, double density_ncr, double sigma_u,
+                        uint64_t seed, int ilo, int ihi, double *rec8, int32_t *id);
+
+/* single-particle kernels, exposed for closed-form tests */
+void oq_push(int pusher, double cmr, double dt, const double e[3], const double b[3],
+             double u[3], double *g);
+double oq_chi(double e_s, const double e[3], const double b[3], const double u[3], double g);
+double oq_rate_w(double e_s, double g, double *r, double chi);       /* mc.cpp:40-74 */
+double oq_rate_tilde_w(double e_s, double g, double r, double chi);  /* mc.cpp:76-97 */
+double oq_vacuum_rate(double e_s, const double e[3], const double b[3]); /* mc.cpp:99-116 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

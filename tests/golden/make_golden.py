@@ -1,0 +1,176 @@
+"""Generate the golden fixtures of tests/golden from the compiled reference.
+
+Runs oracle/_ref/quill_ref (the reference's own hot-path translation units,
+built by oracle/Makefile from the reference tree) on small seeded cases and
+stores inputs plus the reference's outputs as .npz files next to this script.
+Run in the build container only (the reference tree is not on the GPU host):
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Cases
+  run_*      full steps (src/main.cpp:2319-2332 order) from random fields and a
+             mixed particle population (electrons, positrons, ions, photons;
+             QED off); outputs E, B, J, Be, particles (by id) and per-step
+             diagnostics
+  laser_cos  f_init_cos (src/init.cpp:6-210), sscos = 0, tilted and rotated
+  rates      chi / w / tilde_w / mathcal_W (src/mc.cpp:26-116) on a table
+  push       push_boris / push_vay (src/pusher.cpp) on a table
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+import golden_io as gio  # noqa: E402
+import oracle  # noqa: E402
+
+PI = np.pi
+LAMBDA_CM = 1e-4
+E_S = LAMBDA_CM / 2.4263086e-10   # src/main.cpp:2276 (Sauter field for lambda = 1 um)
+PROTON_MASS = 1836.1526721        # include/physical_constants.h:4
+
+
+def norms(dx, dy, dz):
+    """compute_N / compute_energy multipliers of src/main.cpp:2326-2327."""
+    v = dx * dy * dz
+    return dict(norm_n=v * 1.11485e13 * LAMBDA_CM / (8 * PI ** 3),
+                norm_ef=0.5 * v * 3.691e4 * LAMBDA_CM / 1e7,
+                norm_ep=8.2e-14 * v * 1.11485e13 * LAMBDA_CM / (8 * PI ** 3))
+
+
+def run_ref(h, *arrays):
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        gio.write_input(fi, h, *arrays)
+        subprocess.run([oracle.REF_BIN, fi, fo], check=True, stdout=subprocess.DEVNULL)
+        if int(h["mode"]) == gio.MODE_RUN:
+            return gio.read_output_run(fo, h)
+        if int(h["mode"]) == gio.MODE_LASER:
+            return gio.read_output_fields(fo, h)
+        width = 5 if int(h["mode"]) == gio.MODE_RATES else 8
+        return gio.read_output_table(fo, width)
+
+
+def mixed_particles(nx, ny, nz, seed, sigma, ppc=2):
+    """Electrons from the shared loader plus positrons, ions and photons."""
+    rec, ids = oracle.synth_plasma(nx, ny, nz, ppc, 0.5, sigma, seed)
+    rng = np.random.default_rng(seed + 17)
+    extra = []
+    icmr = 1.0 / PROTON_MASS
+    for cmr, count in ((1.0, 40), (icmr, 40), (0.0, 30)):
+        pick = rng.choice(len(rec), count, replace=False)
+        r = rec[pick].copy()
+        r[:, 3:6] = rng.normal(0, sigma, (count, 3))
+        r[:, 6] = 0.25 if cmr != 0 else 0.1
+        r[:, 7] = cmr
+        extra.append(r)
+    ex = np.concatenate(extra)
+    ex_ids = np.arange(len(ex), dtype=np.int32) + 10_000_000
+    return np.concatenate([rec, ex]), np.concatenate([ids, ex_ids]), icmr
+
+
+def case_run(name, nx, ny, nz, dx, dy, dz, dt, solver, pusher, nsteps, seed, sigma, amp,
+             external_bz=0.0, fields=None):
+    rng = np.random.default_rng(seed)
+    shape = (nx, ny, nz, 3)
+    if fields is None:
+        E = rng.uniform(-amp, amp, shape)
+        B = rng.uniform(-amp, amp, shape)
+    else:
+        E, B = fields
+    rec, ids, icmr = mixed_particles(nx, ny, nz, seed, sigma)
+    h = gio.header(mode=gio.MODE_RUN, nx=nx, ny=ny, nz=nz, solver=solver, pusher=pusher, qed=0,
+                   nsteps=nsteps, n_particles=len(rec), seed=seed, n_ion=1, dx=dx, dy=dy, dz=dz, dt=dt,
+                   e_s=E_S, external_bz=external_bz, **norms(dx, dy, dz))
+    h["ion_cmr"][0] = icmr
+    (Eo, Bo, Jo, BEo), prec, pids, diag = run_ref(h, E, B, rec, ids.astype(np.int32))
+    order = np.argsort(pids, kind="stable")
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), header=h.tobytes(), E_in=E, B_in=B, part_in=rec,
+                        ids_in=ids, E=Eo, B=Bo, J=Jo, Be=BEo, part=prec[order], ids=pids[order], diag=diag)
+    print(f"{name}: {len(rec)} particles in, {len(prec)} out, {nsteps} steps")
+
+
+def case_laser():
+    nx, ny, nz = 24, 16, 16
+    dx, dy, dz = 0.1 * PI, 0.12 * PI, 0.1 * PI
+    L = np.zeros(16)
+    # a0y a0z xsigma ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt
+    L[:] = [1.0, 0.3, 0.8 * PI, 1.0 * PI, 1.2 * PI, 12 * dx - 1.0, 1, -1.0, 0.3, 0.2, -0.1, 1, 0.4, 0.0, 0.0, 0.0]
+    h = gio.header(mode=gio.MODE_LASER, nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, dt=0.5 * dx, laser=L)
+    E, B = run_ref(h)
+    L2 = L.copy()
+    L2[6] = 0       # b_sign = -1 branch
+    L2[11] = 1
+    L2[12] = 0.0    # no rotation
+    L2[9] = L2[10] = 0.0
+    h2 = gio.header(mode=gio.MODE_LASER, nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, dt=0.5 * dx, laser=L2)
+    E2, B2 = run_ref(h2)
+    np.savez_compressed(os.path.join(HERE, "laser_cos.npz"), header=h.tobytes(), header2=h2.tobytes(),
+                        E=E, B=B, E2=E2, B2=B2)
+    print("laser_cos: done")
+
+
+def case_rates():
+    rng = np.random.default_rng(7)
+    n = 240
+    g = 10 ** rng.uniform(0, 4.5, n)
+    r = rng.uniform(0, 1, n)
+    chi = 10 ** rng.uniform(-3, 1.3, n)
+    e = rng.normal(0, 1, (n, 3)) * 10 ** rng.uniform(-1, 5.5, (n, 1))
+    b = rng.normal(0, 1, (n, 3)) * 10 ** rng.uniform(-1, 5.5, (n, 1))
+    u = rng.normal(0, 1, (n, 3)) * g[:, None] / np.sqrt(3)
+    # special rows: r = 0, chi = 0, (1-r) g < 1, r g < 1, chi near the 0.133 switch, B = 0, E || B
+    g[:8] = [10, 10, 1.5, 1.5, 100, 100, 50, 50]
+    r[:8] = [0, 0.5, 0.6, 0.5, 0.3, 0.3, 0.2, 0.9]
+    chi[:8] = [0.5, 0, 0.5, 0.5, 0.1329, 0.1331, 0.01, 2.0]
+    b[8] = 0
+    b[9] = e[9] * 0.5
+    e[10] = 0
+    tab = np.column_stack([g, r, chi, e, b, u])
+    h = gio.header(mode=gio.MODE_RATES, nx=4, ny=4, nz=4, dx=0.1, dy=0.1, dz=0.1, dt=0.05, e_s=E_S,
+                   n_particles=n)
+    out = run_ref(h, tab)
+    np.savez_compressed(os.path.join(HERE, "rates.npz"), header=h.tobytes(), table=tab, out=out)
+    print("rates: done")
+
+
+def case_push():
+    rng = np.random.default_rng(11)
+    n = 240
+    cmr = rng.choice([-1.0, 1.0, 1 / PROTON_MASS, -0.5], n)
+    dt = rng.uniform(0.01, 0.5, n)
+    e = rng.normal(0, 1, (n, 3)) * 10 ** rng.uniform(-2, 3, (n, 1))
+    b = rng.normal(0, 1, (n, 3)) * 10 ** rng.uniform(-2, 3, (n, 1))
+    u = rng.normal(0, 1, (n, 3)) * 10 ** rng.uniform(-3, 3, (n, 1))
+    g = np.sqrt(1 + (u ** 2).sum(1))
+    tab = np.column_stack([cmr, dt, e, b, u, g])
+    h = gio.header(mode=gio.MODE_PUSH, nx=4, ny=4, nz=4, n_particles=n)
+    out = run_ref(h, tab)
+    np.savez_compressed(os.path.join(HERE, "push.npz"), header=h.tobytes(), table=tab, out=out)
+    print("push: done")
+
+
+def main():
+    if not os.path.exists(oracle.REF_BIN):
+        sys.exit("oracle/_ref/quill_ref missing: run `make -C oracle` with the reference tree present")
+    dx = 0.05 * 2 * PI
+    dt = 0.025 * 2 * PI
+    case_run("run_fdtd_boris", 14, 12, 10, dx, dx, dx, dt, oracle.FDTD, oracle.BORIS, 10, 101, 0.8, 0.5)
+    case_run("run_ndfx_vay", 12, 14, 10, dx, 1.2 * dx, 1.4 * dx, 0.9 * dt, oracle.NDFX, oracle.VAY, 10, 202,
+             0.8, 0.5, external_bz=0.3)
+    case_run("run_five_vay", 12, 10, 12, dx, dx, dx, dt, oracle.FIVE_POINT, oracle.VAY, 10, 303, 0.8, 0.5)
+    case_run("run_fdtd_vay_cold", 12, 12, 12, dx, dx, dx, dt, oracle.FDTD, oracle.VAY, 12, 404, 0.01, 0.05)
+    case_laser()
+    case_rates()
+    case_push()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,115 @@
+"""Pin the FP64 oracle against golden vectors of the compiled reference.
+
+The fixtures in tests/golden were produced by oracle/_ref/quill_ref, which is
+the reference's own src/*.cpp hot-path units linked with a single-rank driver
+(tests/golden/make_golden.py).  Every oracle function on the path is checked
+here: pushers, rate functions, the laser initial condition, and full steps of
+all three solvers (gather, interpolate_be, jdeposition, boundaries, place,
+diagnostics) on a mixed population of electrons, positrons, ions and photons.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import golden_io as gio  # noqa: E402
+import oracle  # noqa: E402
+
+RUN_CASES = ["run_fdtd_boris", "run_ndfx_vay", "run_five_vay", "run_fdtd_vay_cold"]
+
+
+def hdr(fx, key="header"):
+    return np.frombuffer(fx[key].tobytes(), gio.HDR)[0]
+
+
+def make_oracle(h):
+    return oracle.Oracle(int(h["nx"]), int(h["ny"]), int(h["nz"]), float(h["dx"]), float(h["dy"]),
+                         float(h["dz"]), float(h["dt"]), solver=int(h["solver"]), pusher=int(h["pusher"]),
+                         qed=bool(h["qed"]), e_s=float(h["e_s"]), external_bz=float(h["external_bz"]),
+                         q_vac=float(h["q_vac"]), seed=int(h["seed"]), norm_n=float(h["norm_n"]),
+                         norm_ef=float(h["norm_ef"]), norm_ep=float(h["norm_ep"]))
+
+
+def rel_err(a, b):
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+@pytest.mark.parametrize("name", RUN_CASES)
+def test_run_matches_reference(name):
+    fx = gio.load(name)
+    h = hdr(fx)
+    o = make_oracle(h)
+    o.field(oracle.E)[:] = fx["E_in"]
+    o.field(oracle.B)[:] = fx["B_in"]
+    o.add_particles(fx["part_in"], fx["ids_in"])
+    o.init_boundaries()
+    diag = o.step(int(h["nsteps"]), diag=True)
+    for key, which in (("E", oracle.E), ("B", oracle.B), ("J", oracle.J), ("Be", oracle.BE)):
+        err = rel_err(o.field(which), fx[key])
+        assert err < 1e-11, f"{name} {key}: rel err {err:.3e}"
+    rec, ids = o.particles()
+    order = np.argsort(ids, kind="stable")
+    rec, ids = rec[order], ids[order]
+    np.testing.assert_array_equal(ids, fx["ids"])
+    np.testing.assert_allclose(rec[:, :3], fx["part"][:, :3], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(rec[:, 3:6], fx["part"][:, 3:6], rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(rec[:, 6], fx["part"][:, 6], rtol=1e-12)
+    np.testing.assert_array_equal(rec[:, 7:9], fx["part"][:, 7:9])
+    ref = fx["diag"]
+    for l, d in enumerate(diag):
+        mine = [d.N_e, d.N_p, d.N_ph, d.n_e, d.n_p, d.n_ph, d.energy_f, d.energy_e, d.energy_p,
+                d.energy_ph, d.energy_ion, d.n_ion]
+        np.testing.assert_allclose(mine, ref[l], rtol=1e-11, atol=0, err_msg=f"{name} step {l}")
+
+
+def test_laser_matches_reference():
+    fx = gio.load("laser_cos")
+    for hk, ek, bk in (("header", "E", "B"), ("header2", "E2", "B2")):
+        h = hdr(fx, hk)
+        o = make_oracle(h)
+        L = h["laser"]
+        o.laser_cos(L[0], L[1], L[2], L[3], L[4], L[5], int(L[6]), L[7], L[8], L[9], L[10], bool(L[11]), L[12],
+                    L[13], L[14], L[15])
+        assert rel_err(o.field(oracle.E), fx[ek]) < 1e-12
+        assert rel_err(o.field(oracle.B), fx[bk]) < 1e-12
+        assert np.abs(fx[ek]).max() > 0.1
+
+
+def test_rates_match_reference():
+    fx = gio.load("rates")
+    h = hdr(fx)
+    e_s = float(h["e_s"])
+    L = oracle.lib()
+    for row, out in zip(fx["table"], fx["out"]):
+        g, r, chi = row[0], row[1], row[2]
+        e, b, u = (np.ascontiguousarray(row[3:6]), np.ascontiguousarray(row[6:9]),
+                   np.ascontiguousarray(row[9:12]))
+        c = L.oq_chi(e_s, oracle._dp(e), oracle._dp(b), oracle._dp(u), g)
+        assert c == pytest.approx(out[0], rel=1e-12, abs=1e-300) or (np.isnan(c) and np.isnan(out[0]))
+        rr = np.array([r])
+        w = L.oq_rate_w(e_s, g, oracle._dp(rr), chi)
+        assert w == pytest.approx(out[1], rel=1e-11, abs=0)
+        assert rr[0] == pytest.approx(out[2], rel=1e-14, abs=0)
+        tw = L.oq_rate_tilde_w(e_s, g, r, chi)
+        assert tw == pytest.approx(out[3], rel=1e-11, abs=0)
+        vw = L.oq_vacuum_rate(e_s, oracle._dp(e), oracle._dp(b))
+        assert vw == pytest.approx(out[4], rel=1e-11, abs=0)
+    # the table exercises the nonzero branches too
+    assert (fx["out"][:, 1] != 0).sum() > 50 and (fx["out"][:, 3] != 0).sum() > 50
+    assert (fx["out"][:, 4] > 0).sum() > 5
+
+
+def test_pushers_match_reference():
+    fx = gio.load("push")
+    for row, out in zip(fx["table"], fx["out"]):
+        cmr, dt, e, b, u, g = row[0], row[1], row[2:5], row[5:8], row[8:11], row[11]
+        ub, gb = oracle.push(oracle.BORIS, cmr, dt, e, b, u, g)
+        uv, gv = oracle.push(oracle.VAY, cmr, dt, e, b, u, g)
+        np.testing.assert_allclose(np.r_[ub, gb], out[:4], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(np.r_[uv, gv], out[4:], rtol=1e-12, atol=1e-14)
