@@ -1,0 +1,238 @@
+"""Benchmark of the per-step PIC loop on one MI355X.
+
+Workload (default "uniform-ndfx-vay-256x128x128"): 256 x 128 x 128 cells,
+dx = dy = dz = 0.05 lambda, dt = 0.025 lambda/c, NDFX solver, Vay pusher, QED
+off, uniform electron plasma at 0.01 n_cr with 8 particles per cell (is_inside
+cells only), thermal momenta N(0, 0.01^2), linearly polarised laser a0 = 1,
+waist 2 lambda, duration 4 lambda/c set as the reference's cos-envelope
+initial condition at x-min.  FP32 on the GPU.
+
+A "step" is one pass of src/main.cpp:2319-2332 (padvance, compute_N,
+compute_energy, fadvance) over the whole subdomain.  The timed region holds K
+steps with every input resident in HBM; it is bracketed by stream
+synchronisation of the library's stream (all work of a step is enqueued
+there).  torch is not imported: the library brings its own HIP runtime.
+
+Printed: one JSON line with value = particle-steps/s, the roofline of the
+dominant kernel (the fused gather/push/deposit kernel) from HIP events on the
+library's stream, the plain FP64 GPU method on the same workload, the FP32 vs
+FP64 field error, and the reference CPU path timed on this host.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import quill_amd as qa  # noqa: E402
+
+TWO_PI = 2 * math.pi
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s on 1 MI355X (push+deposit+Maxwell), % HBM roofline vs FP64"
+
+WORKLOADS = {
+    # name: grid, solver, pusher, density (n_cr), laser a0, waist (lambda), duration (lambda/c)
+    "uniform-ndfx-vay-256x128x128": dict(grid=(256, 128, 128), solver="ndfx", pusher="vay", density=0.01, a0=1.0,
+                                         waist=2.0, duration=4.0),
+    "uniform-fdtd-boris-64x32x32": dict(grid=(64, 32, 32), solver="fdtd", pusher="boris", density=0.01, a0=1.0,
+                                        waist=2.0, duration=4.0),
+    "wakefield-fdtd-boris-512x256x256": dict(grid=(512, 256, 256), solver="fdtd", pusher="boris", density=0.001,
+                                             a0=4.0, waist=6.0, duration=8.0),
+}
+PPC = 8
+SIGMA_U = 0.01
+SEED = 20261202
+DX_LAMBDA = 0.05
+DT_LAMBDA = 0.025
+
+
+def laser_params(w, dx):
+    """f_init_cos arguments for a pulse whose rear edge sits at x-min, moving +x.
+
+    xsigma / ysigma are the 1/e^2 half sizes the reference's cos envelope is
+    matched to (src/init.cpp:34-37): xsigma = duration / 2, ysigma = zsigma =
+    waist.  x0 places the pulse centre one envelope half-length xs from the
+    left wall; x1 < 0 gives propagation along +x (src/init.cpp:56-59)."""
+    xsigma = w["duration"] / 2 * TWO_PI
+    ysig = w["waist"] * TWO_PI
+    xs = xsigma * 2 * math.sqrt(2 * math.pi) / 3
+    nx = w["grid"][0]
+    x0 = xs
+    x1 = x0 - nx * dx / 2   # pulse centre relative to the box centre
+    if x1 >= 0:
+        x1 = -1e-6
+    # a0y a0z xsigma ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt
+    return dict(a0y=w["a0"], a0z=0.0, xsigma=xsigma, ysigma=ysig, zsigma=ysig, x0=x0, b_sign=1, x1=x1)
+
+
+def make_region(w, precision, deposit, sort_every=20):
+    nx, ny, nz = w["grid"]
+    dx = DX_LAMBDA * TWO_PI
+    dt = DT_LAMBDA * TWO_PI
+    r = qa.Region(nx, ny, nz, dx, dx, dx, dt, solver=w["solver"], pusher=w["pusher"], precision=precision,
+                  deposit=deposit, sort_every=sort_every, device=0)
+    n = r.load_uniform_plasma(qa.ELECTRON, PPC, w["density"], SIGMA_U, SEED)
+    r.init_laser_cos(**laser_params(w, dx))
+    r.init_boundaries()
+    return r, n
+
+
+def timed_run(w, precision, deposit, steps, warmup):
+    r, n0 = make_region(w, precision, deposit)
+    if warmup:
+        r.step(warmup)
+    r.sync()
+    n_start = r.count(qa.ELECTRON)
+    t0 = time.perf_counter()
+    diag = r.step(steps, diag=True)
+    r.sync()
+    t1 = time.perf_counter()
+    ms_total, ms_part, ms_field = r.timing()
+    # particles pushed in step l are those alive before it
+    pushed = n_start + sum(d["n_e"] for d in diag[:-1])
+    r.close()
+    return dict(seconds=t1 - t0, pushed=pushed, n_start=n_start, n_loaded=n0, ms_events=ms_total,
+                ms_particles=ms_part, ms_fields=ms_field, energy_f_last=diag[-1]["energy_f"])
+
+
+def field_error(w, steps):
+    """Per-component relative L2 error of FP32 vs the plain FP64 method after `steps`."""
+    out = {}
+    res = {}
+    for prec, dep in (("fp32", "tiled"), ("fp64", "global")):
+        r, _ = make_region(w, prec, dep)
+        r.step(steps)
+        res[prec] = r.get_fields()
+        r.close()
+    for name, a, b in zip(("E", "B", "J"), res["fp32"][:3], res["fp64"][:3]):
+        for c, axis in enumerate("xyz"):
+            den = np.linalg.norm(b[..., c])
+            out[f"{name}{axis}"] = float(np.linalg.norm(a[..., c] - b[..., c]) / den) if den > 0 else 0.0
+    return out
+
+
+def cpu_baseline(w, budget_steps=2):
+    """The reference's own step (oracle/_ref/quill_ref: the reference's
+    src/*.cpp units, single-threaded as the reference runs per rank) on the
+    same workload: full grid, one warm-up step, then `budget_steps` timed
+    steps.  Falls back to the FP64 C restatement when the reference build is
+    absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import golden_io as gio
+    import oracle
+
+    nx, ny, nz = w["grid"]
+    dx = DX_LAMBDA * TWO_PI
+    dt = DT_LAMBDA * TWO_PI
+    lp = laser_params(w, dx)
+    L = np.zeros(16)
+    L[:8] = [lp["a0y"], lp["a0z"], lp["xsigma"], lp["ysigma"], lp["zsigma"], lp["x0"], lp["b_sign"], lp["x1"]]
+    L[11] = 1
+    solver = {"fdtd": 0, "ndfx": 1, "five_point": 2}[w["solver"]]
+    pusher = {"vay": 0, "boris": 1}[w["pusher"]]
+    cores = 1
+    if os.path.exists(oracle.REF_BIN):
+        h = gio.header(mode=gio.MODE_TIME, nx=nx, ny=ny, nz=nz, solver=solver, pusher=pusher, nsteps=budget_steps,
+                       seed=SEED & 0x7FFFFFFF, dx=dx, dy=dx, dz=dx, dt=dt, laser=L)
+        h["extra"][:6] = [PPC, w["density"], SIGMA_U, 1, 1, nx]
+        with tempfile.TemporaryDirectory() as td:
+            fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+            gio.write_input(fi, h)
+            res = subprocess.run([oracle.REF_BIN, fi, fo], capture_output=True, text=True, check=True)
+        rec = json.loads(res.stdout.strip().splitlines()[-1])
+        rate = rec["particles"] * rec["steps"] / rec["seconds"]
+        return dict(value=rate, unit="particle-steps/s", cores=cores, kind="reference",
+                    sample=f"reference src/*.cpp step (oracle/_ref), full {nx}x{ny}x{nz} grid, "
+                           f"{rec['particles']} particles, 1 warm-up + {rec['steps']} timed steps, "
+                           f"{rec['seconds']:.2f} s, single thread")
+    o = oracle.Oracle(nx, ny, nz, dx, dx, dx, dt, solver=solver, pusher=pusher)
+    n = o.load_uniform(PPC, w["density"], SIGMA_U, SEED)
+    o.laser_cos(*L)
+    o.init_boundaries()
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(budget_steps)
+    sec = time.perf_counter() - t0
+    return dict(value=n * budget_steps / sec, unit="particle-steps/s", cores=cores, kind="port",
+                sample=f"FP64 C restatement (oracle/), full grid, {n} particles, {budget_steps} steps, {sec:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="uniform-ndfx-vay-256x128x128", choices=sorted(WORKLOADS))
+    ap.add_argument("--fp64-steps", type=int, default=50, help="steps of the plain FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU slab decomposition is a design, see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.workload]
+    nx, ny, nz = w["grid"]
+    ncell = nx * ny * nz
+
+    main_run = timed_run(w, "fp32", "tiled", args.steps, args.warmup)
+    K = args.steps
+    ms_step = main_run["seconds"] * 1e3 / K
+    value = main_run["pushed"] / main_run["seconds"]
+    np_avg = main_run["pushed"] / K
+    # dominant kernel: gather + push + deposit, one launch per step (electrons only)
+    push_ms = main_run["ms_particles"] / K
+    push_bytes = 64.0 * np_avg + 36.0 * ncell
+    achieved = push_bytes / (push_ms * 1e-3) / 1e9
+    step_bytes = 64.0 * np_avg + 96.0 * ncell
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": [nx, ny, nz], "cells": ncell, "ppc": PPC,
+                   "particles": main_run["n_start"], "solver": w["solver"], "pusher": w["pusher"], "qed": False,
+                   "deposit": "tiled", "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_push (gather+push+deposit)", "avg_launch_ms": push_ms,
+                     "algorithmic_bytes_per_launch": push_bytes},
+        "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
+                     "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
+                     "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "cell_updates_per_s": ncell * K / main_run["seconds"],
+        "ms_particles_per_step": push_ms,
+        "ms_fields_per_step": main_run["ms_fields"] / K,
+    }
+    if not args.no_compare:
+        k64 = min(args.fp64_steps, K)
+        plain = timed_run(w, "fp64", "global", k64, 2)
+        v64 = plain["pushed"] / plain["seconds"]
+        line["fp64_plain"] = {"value": v64, "unit": "particle-steps/s", "steps": k64,
+                              "ms_per_step": plain["seconds"] * 1e3 / k64, "speedup_fp32": value / v64,
+                              "method": "FP64 SoA, one thread per particle, global FP64 atomics"}
+        line["fp32_vs_fp64_rel_l2"] = field_error(w, min(k64, 50))
+        line["fp32_vs_fp64_steps"] = min(k64, 50)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

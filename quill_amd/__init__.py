@@ -1,0 +1,273 @@
+"""quill_amd -- MI355X-native per-step PIC loop of QUILL, Python host binding.
+
+The compute path is libquillk.so (HIP kernels for gfx950 behind the C ABI of
+include/quillk.h).  This module binds that ABI with ctypes and mirrors the
+reference's spatial_region interface (include/main.h:17-170): create the
+subdomain, set fields, place particles, f_init_boundaries / interpolate_be,
+and step in the order of src/main.cpp:2319-2332.
+
+There is no fallback: if the library is missing or no GPU is present, the
+calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libquillk.so")
+
+QK_OK = 0
+ERRORS = {1: "invalid argument", 2: "capacity exceeded", 3: "HIP error", 4: "bad state", 5: "not built"}
+
+SOLVERS = {"fdtd": 0, "ndfx": 1, "five_point": 2}
+PUSHERS = {"vay": 0, "boris": 1}
+PRECISIONS = {"fp32": 0, "fp64": 1}
+DEPOSITS = {"tiled": 0, "global": 1}
+ELECTRON, POSITRON, PHOTON, ION0 = 0, 1, 2, 3
+MAX_ION = 4
+MAX_SPECIES = ION0 + MAX_ION
+SPECIES_NAMES = {"electron": ELECTRON, "positron": POSITRON, "photon": PHOTON}
+
+EXPORTS = ["qk_create", "qk_destroy", "qk_last_error", "qk_set_fields", "qk_get_fields", "qk_add_particles",
+           "qk_load_uniform_plasma", "qk_init_laser_cos", "qk_init_boundaries", "qk_step", "qk_sync",
+           "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_get_stream"]
+
+
+class QuillError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """qk_params of include/quillk.h."""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+                ("solver", C.c_int32), ("pusher", C.c_int32), ("qed", C.c_int32), ("precision", C.c_int32),
+                ("deposit", C.c_int32), ("sort_every", C.c_int32), ("track_ids", C.c_int32),
+                ("device", C.c_int32),
+                ("e_s", C.c_double), ("external_bz", C.c_double), ("q_vac", C.c_double),
+                ("seed", C.c_uint64), ("capacity", C.c_int64),
+                ("n_ion", C.c_int32), ("ion_cmr", C.c_double * MAX_ION),
+                ("norm_n", C.c_double), ("norm_ef", C.c_double), ("norm_ep", C.c_double)]
+
+
+class Diag(C.Structure):
+    """qk_diag of include/quillk.h (compute_N / compute_energy, src/fout.cpp:41-138)."""
+    _fields_ = [("N_e", C.c_double), ("N_p", C.c_double), ("N_ph", C.c_double),
+                ("n_e", C.c_int64), ("n_p", C.c_int64), ("n_ph", C.c_int64), ("n_ion", C.c_int64),
+                ("energy_f", C.c_double), ("energy_e", C.c_double), ("energy_p", C.c_double),
+                ("energy_ph", C.c_double), ("energy_ion", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libquillk.so and declare the ABI; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise QuillError(f"{path} not found: build it with `make -C quill_amd/csrc` or __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp, dp, ip, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    L.qk_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.qk_destroy.argtypes = [vp]
+    L.qk_last_error.restype = C.c_char_p
+    L.qk_last_error.argtypes = []
+    L.qk_set_fields.argtypes = [vp, dp, dp]
+    L.qk_get_fields.argtypes = [vp, dp, dp, dp, dp]
+    L.qk_add_particles.argtypes = [vp, C.c_int, C.c_int64] + [dp] * 7 + [ip, i64p]
+    L.qk_load_uniform_plasma.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_int,
+                                         C.c_int, i64p]
+    L.qk_init_laser_cos.argtypes = [vp, dp]
+    L.qk_init_boundaries.argtypes = [vp]
+    L.qk_step.argtypes = [vp, C.c_int, C.POINTER(Diag)]
+    L.qk_sync.argtypes = [vp]
+    L.qk_particle_count.argtypes = [vp, C.c_int, i64p]
+    L.qk_get_particles.argtypes = [vp, C.c_int, C.c_int64, dp, ip, i64p]
+    L.qk_compute_rho.argtypes = [vp, dp]
+    L.qk_last_timing.argtypes = [vp, dp, dp, dp]
+    L.qk_get_stream.argtypes = [vp, C.POINTER(vp)]
+    for name in EXPORTS:
+        if name not in ("qk_last_error",):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != QK_OK:
+        msg = load_library().qk_last_error().decode(errors="replace")
+        raise QuillError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Region:
+    """One subdomain on one GPU: the reference's spatial_region (include/main.h:17-170).
+
+    Parameters follow qk_params; solver / pusher / precision / deposit are
+    given by name.  Fields are numpy arrays (nx, ny, nz, 3) in FP64 on the
+    host side of the boundary.
+    """
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, dt, solver="fdtd", pusher="boris", precision="fp32",
+                 deposit="tiled", qed=False, sort_every=0, track_ids=False, device=0, e_s=1e10,
+                 external_bz=0.0, q_vac=0.0, seed=1, capacity=0, ion_cmr=(), norm_n=1.0, norm_ef=1.0,
+                 norm_ep=1.0):
+        L = load_library()
+        p = Params()
+        p.nx, p.ny, p.nz = nx, ny, nz
+        p.dx, p.dy, p.dz, p.dt = dx, dy, dz, dt
+        p.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
+        p.pusher = PUSHERS[pusher] if isinstance(pusher, str) else int(pusher)
+        p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        p.deposit = DEPOSITS[deposit] if isinstance(deposit, str) else int(deposit)
+        p.qed = int(bool(qed))
+        p.sort_every = sort_every
+        p.track_ids = int(bool(track_ids))
+        p.device = device
+        p.e_s, p.external_bz, p.q_vac = e_s, external_bz, q_vac
+        p.seed = seed
+        p.capacity = capacity
+        p.n_ion = len(ion_cmr)
+        for n, v in enumerate(ion_cmr):
+            p.ion_cmr[n] = v
+        p.norm_n, p.norm_ef, p.norm_ep = norm_n, norm_ef, norm_ep
+        self.params = p
+        self.shape = (nx, ny, nz)
+        self.precision = precision
+        h = C.c_void_p()
+        _check(L.qk_create(C.byref(p), C.byref(h)), "qk_create")
+        self._h = h
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(load_library().qk_destroy(self._h), "qk_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- fields -------------------------------------------------------------
+    def set_fields(self, E=None, B=None):
+        e = np.ascontiguousarray(E, np.float64) if E is not None else None
+        b = np.ascontiguousarray(B, np.float64) if B is not None else None
+        for a in (e, b):
+            if a is not None and a.shape != self.shape + (3,):
+                raise ValueError(f"field shape {a.shape} != {self.shape + (3,)}")
+        _check(load_library().qk_set_fields(self._h, _dp(e) if e is not None else None,
+                                            _dp(b) if b is not None else None), "qk_set_fields")
+
+    def get_fields(self):
+        """Return (E, B, J, Be) as (nx, ny, nz, 3) FP64 arrays."""
+        out = [np.zeros(self.shape + (3,)) for _ in range(4)]
+        _check(load_library().qk_get_fields(self._h, *[_dp(a) for a in out]), "qk_get_fields")
+        return tuple(out)
+
+    def init_laser_cos(self, a0y, a0z, xsigma, ysigma, zsigma, x0, b_sign=1, x1=0.0, phase=0.0, y0=0.0, z0=0.0,
+                       append=True, phi=0.0, xtarget=0.0, ytarget=0.0, ztarget=0.0):
+        L = np.array([a0y, a0z, xsigma, ysigma, zsigma, x0, float(b_sign), x1, phase, y0, z0, float(append), phi,
+                      xtarget, ytarget, ztarget], np.float64)
+        _check(load_library().qk_init_laser_cos(self._h, _dp(L)), "qk_init_laser_cos")
+
+    def init_boundaries(self):
+        _check(load_library().qk_init_boundaries(self._h), "qk_init_boundaries")
+
+    # -- particles ----------------------------------------------------------
+    def add_particles(self, species, x, y, z, ux, uy, uz, q, ids=None):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, ux, uy, uz, q)]
+        n = len(arrs[0])
+        idv = np.ascontiguousarray(ids, np.int32) if ids is not None else None
+        kept = C.c_int64()
+        _check(load_library().qk_add_particles(self._h, species, n, *[_dp(a) for a in arrs],
+                                               _ip(idv) if idv is not None else None, C.byref(kept)),
+               "qk_add_particles")
+        return kept.value
+
+    def add_records(self, rec, ids=None, species_of_cmr=None):
+        """Add rows x y z ux uy uz q cmr, choosing the species from cmr."""
+        rec = np.asarray(rec, np.float64)
+        kept = 0
+        for cmr in np.unique(rec[:, 7]):
+            if species_of_cmr is not None:
+                s = species_of_cmr(cmr)
+            elif cmr == -1:
+                s = ELECTRON
+            elif cmr == 1:
+                s = POSITRON
+            elif cmr == 0:
+                s = PHOTON
+            else:
+                cm = list(self.params.ion_cmr[:self.params.n_ion])
+                s = ION0 + cm.index(cmr)
+            sel = rec[:, 7] == cmr
+            kept += self.add_particles(s, *[rec[sel, c] for c in range(7)],
+                                       ids=ids[sel] if ids is not None else None)
+        return kept
+
+    def load_uniform_plasma(self, species, ppc, density, sigma_u, seed, ilo=1, ihi=1 << 30):
+        added = C.c_int64()
+        _check(load_library().qk_load_uniform_plasma(self._h, species, ppc, density, sigma_u, seed, ilo, ihi,
+                                                     C.byref(added)), "qk_load_uniform_plasma")
+        return added.value
+
+    def count(self, species):
+        n = C.c_int64()
+        _check(load_library().qk_particle_count(self._h, species, C.byref(n)), "qk_particle_count")
+        return n.value
+
+    def particles(self, species):
+        """Return (rec[n,10] = x y z ux uy uz g q cmr chi, ids[n])."""
+        cap = max(self.count(species), 1)
+        rec = np.zeros((cap, 10))
+        ids = np.zeros(cap, np.int32)
+        n = C.c_int64()
+        _check(load_library().qk_get_particles(self._h, species, cap, _dp(rec), _ip(ids), C.byref(n)),
+               "qk_get_particles")
+        return rec[:n.value], ids[:n.value]
+
+    def compute_rho(self):
+        rho = np.zeros(self.shape + (3,))
+        _check(load_library().qk_compute_rho(self._h, _dp(rho)), "qk_compute_rho")
+        return rho
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, nsteps=1, diag=False):
+        d = (Diag * max(nsteps, 1))() if diag else None
+        _check(load_library().qk_step(self._h, nsteps, d), "qk_step")
+        return [d[i].as_dict() for i in range(nsteps)] if diag else None
+
+    def sync(self):
+        _check(load_library().qk_sync(self._h), "qk_sync")
+
+    def timing(self):
+        """(ms_total, ms_particles, ms_fields) of the last step() call, from HIP events."""
+        t = [C.c_double(), C.c_double(), C.c_double()]
+        _check(load_library().qk_last_timing(self._h, *[C.byref(v) for v in t]), "qk_last_timing")
+        return tuple(v.value for v in t)
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(load_library().qk_get_stream(self._h, C.byref(s)), "qk_get_stream")
+        return s.value

@@ -1,0 +1,680 @@
+// qk_api.hip -- the C ABI of include/quillk.h: context (the reference's
+// spatial_region at n_sr = 1, include/main.h:17-170), device buffers, the
+// step driver in the order of src/main.cpp:2319-2332, and copies in and out.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "quillk.h"
+#include "qk_kernels.h"
+
+using namespace qk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) return fail(QK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct SpeciesBuf {
+    bool active = false;
+    double cmr = 0;
+    int kind = 0;         // 0 charged, 1 photon
+    int energy_mode = 0;  // 0 (g-1) q, 1 g q, 2 (g-1) q / cmr
+    int64_t cap = 0;
+    int64_t n = 0;        // slots in use (live or free)
+    void *mem = nullptr;  // one allocation holding every array
+    int *cell = nullptr;
+    void *ox = nullptr, *oy = nullptr, *oz = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *q = nullptr,
+         *chi = nullptr;
+    int *id = nullptr;
+};
+
+}  // namespace
+
+struct qk_ctx {
+    qk_params p;
+    Grid g;
+    SolverCoef coef;
+    hipStream_t stream = nullptr;
+    size_t rsize = 4;               // bytes per real
+    void *fmem = nullptr;           // all field arrays
+    void *F[15] = {};               // ex ey ez bx by bz bex bey bez j0x j0y j0z j1x j1y j1z
+    int jcur = 0;                   // current buffer deposited this step
+    bool have_step = false;
+    SpeciesBuf sp[QK_MAX_SPECIES];
+    double *d_diag = nullptr;       // [QK_MAX_SPECIES][3] + field energy
+    double *h_diag = nullptr;       // pinned per-step copies of the accumulators
+    size_t h_diag_cap = 0;          // steps that fit in h_diag
+    std::vector<hipEvent_t> ev;     // timing events
+    double ms_total = 0, ms_part = 0, ms_field = 0;
+};
+
+namespace {
+
+template <typename real>
+Fields<real> fields_of(qk_ctx *c)
+{
+    Fields<real> f;
+    f.ex = (real *)c->F[0]; f.ey = (real *)c->F[1]; f.ez = (real *)c->F[2];
+    f.bx = (real *)c->F[3]; f.by = (real *)c->F[4]; f.bz = (real *)c->F[5];
+    f.bex = (real *)c->F[6]; f.bey = (real *)c->F[7]; f.bez = (real *)c->F[8];
+    return f;
+}
+
+template <typename real>
+Current<real> current_of(qk_ctx *c, int which)
+{
+    Current<real> j;
+    j.jx = (real *)c->F[9 + 3 * which];
+    j.jy = (real *)c->F[10 + 3 * which];
+    j.jz = (real *)c->F[11 + 3 * which];
+    return j;
+}
+
+template <typename real>
+Species<real> species_of(SpeciesBuf &b)
+{
+    Species<real> s;
+    s.cell = b.cell;
+    s.ox = (real *)b.ox; s.oy = (real *)b.oy; s.oz = (real *)b.oz;
+    s.ux = (real *)b.ux; s.uy = (real *)b.uy; s.uz = (real *)b.uz;
+    s.q = (real *)b.q;
+    s.chi = (real *)b.chi;
+    s.id = b.id;
+    return s;
+}
+
+double species_cmr(const qk_params &p, int s)
+{
+    if (s == QK_ELECTRON) return -1;
+    if (s == QK_POSITRON) return 1;
+    if (s == QK_PHOTON) return 0;
+    return p.ion_cmr[s - QK_ION0];
+}
+
+bool species_valid(const qk_ctx *c, int s)
+{
+    if (s < 0 || s >= QK_MAX_SPECIES) return false;
+    if (s >= QK_ION0 && s - QK_ION0 >= c->p.n_ion) return false;
+    return true;
+}
+
+// (re)allocate the arrays of one species to hold cap slots, keeping n
+int reserve(qk_ctx *c, int s, int64_t cap)
+{
+    SpeciesBuf &b = c->sp[s];
+    if (cap <= b.cap) return QK_OK;
+    const size_t r = c->rsize;
+    const bool chi = c->p.qed != 0, ids = c->p.track_ids != 0;
+    // per slot: cell (4) + 7 reals + chi + id
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t a_int = align((size_t)cap * 4), a_real = align((size_t)cap * r);
+    const size_t bytes = a_int + 7 * a_real + (chi ? a_real : 0) + (ids ? a_int : 0);
+    void *mem = nullptr;
+    HIP_TRY(hipMalloc(&mem, bytes));
+    char *m = (char *)mem;
+    SpeciesBuf nb = b;
+    nb.mem = mem;
+    nb.cap = cap;
+    nb.cell = (int *)m; m += a_int;
+    nb.ox = m; m += a_real;
+    nb.oy = m; m += a_real;
+    nb.oz = m; m += a_real;
+    nb.ux = m; m += a_real;
+    nb.uy = m; m += a_real;
+    nb.uz = m; m += a_real;
+    nb.q = m; m += a_real;
+    nb.chi = chi ? (void *)m : nullptr; m += chi ? a_real : 0;
+    nb.id = ids ? (int *)m : nullptr;
+    if (b.mem && b.n > 0) {
+        HIP_TRY(hipMemcpyAsync(nb.cell, b.cell, (size_t)b.n * 4, hipMemcpyDeviceToDevice, c->stream));
+        void *src[8] = {b.ox, b.oy, b.oz, b.ux, b.uy, b.uz, b.q, b.chi};
+        void *dst[8] = {nb.ox, nb.oy, nb.oz, nb.ux, nb.uy, nb.uz, nb.q, nb.chi};
+        for (int a = 0; a < 8; ++a)
+            if (src[a] && dst[a]) HIP_TRY(hipMemcpyAsync(dst[a], src[a], (size_t)b.n * r, hipMemcpyDeviceToDevice, c->stream));
+        if (b.id && nb.id) HIP_TRY(hipMemcpyAsync(nb.id, b.id, (size_t)b.n * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (b.mem) HIP_TRY(hipFree(b.mem));
+    b = nb;
+    return QK_OK;
+}
+
+int ensure_species(qk_ctx *c, int s, int64_t extra)
+{
+    SpeciesBuf &b = c->sp[s];
+    if (!b.active) {
+        b.active = true;
+        b.cmr = species_cmr(c->p, s);
+        b.kind = (s == QK_PHOTON) ? 1 : 0;
+        b.energy_mode = (s == QK_PHOTON) ? 1 : (s >= QK_ION0 ? 2 : 0);
+    }
+    const int64_t need = b.n + extra;
+    if (need <= b.cap) return QK_OK;
+    int64_t cap = c->p.capacity > 0 ? c->p.capacity : 2 * need;
+    if (cap < need) {
+        if (c->p.capacity > 0)
+            return fail(QK_ERR_CAPACITY, "particle capacity " + std::to_string(c->p.capacity) + " exceeded (need " +
+                                             std::to_string(need) + ")");
+        cap = need;
+    }
+    return reserve(c, s, cap);
+}
+
+void unpack_diag(const qk_ctx *c, const double *h, qk_diag &d)
+{
+    std::memset(&d, 0, sizeof d);
+    const double nn = c->p.norm_n, ef = c->p.norm_ef, ep = c->p.norm_ep;
+    const double *e = h + QK_ELECTRON * kDiagPerSpecies, *po = h + QK_POSITRON * kDiagPerSpecies,
+                 *ph = h + QK_PHOTON * kDiagPerSpecies;
+    d.N_e = -e[0] * nn;   // electrons count with the opposite sign (fout.cpp:62)
+    d.n_e = (int64_t)e[2];
+    d.energy_e = -e[1] * ep;
+    d.N_p = po[0] * nn;
+    d.n_p = (int64_t)po[2];
+    d.energy_p = po[1] * ep;
+    d.N_ph = ph[0] * nn;
+    d.n_ph = (int64_t)ph[2];
+    d.energy_ph = ph[1] * ep;
+    for (int s = QK_ION0; s < QK_MAX_SPECIES; ++s) {
+        d.n_ion += (int64_t)h[s * kDiagPerSpecies + 2];
+        d.energy_ion += h[s * kDiagPerSpecies + 1] * ep;
+    }
+    d.energy_f = h[QK_MAX_SPECIES * kDiagPerSpecies] * ef;
+}
+
+int ensure_events(qk_ctx *c, size_t n)
+{
+    while (c->ev.size() < n) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    return QK_OK;
+}
+
+constexpr size_t kDiagLen = QK_MAX_SPECIES * kDiagPerSpecies + 1;
+
+template <typename real>
+int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
+{
+    const Fields<real> f = fields_of<real>(c);
+    // three events per step: before the particle phase, between phases, after the field phase
+    int rc = ensure_events(c, 3 * (size_t)nsteps);
+    if (rc) return rc;
+    std::vector<double> hbuf;
+    double *hd = nullptr;
+    if (diag) {
+        if (c->h_diag_cap < (size_t)nsteps) {
+            if (c->h_diag) HIP_TRY(hipHostFree(c->h_diag));
+            c->h_diag = nullptr;
+            HIP_TRY(hipHostMalloc((void **)&c->h_diag, (size_t)nsteps * kDiagLen * sizeof(double)));
+            c->h_diag_cap = (size_t)nsteps;
+        }
+        hd = c->h_diag;
+    }
+    for (int l = 0; l < nsteps; ++l) {
+        const Current<real> jc = current_of<real>(c, c->jcur), jn = current_of<real>(c, c->jcur ^ 1);
+        HIP_TRY(launch_zero(c->stream, c->d_diag, kDiagLen));
+        // padvance (src/advance.cpp:90-387); the boundary erase of p_boundary
+        // (src/plist.cpp:73-102) never finds a particle here: place() keeps only
+        // is_inside cells, which exclude the erased layers
+        HIP_TRY(hipEventRecord(c->ev[3 * l], c->stream));
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            SpeciesBuf &b = c->sp[s];
+            if (!b.active || b.n == 0) continue;
+            PushArgs a{};
+            a.pusher = c->p.pusher;
+            a.kind = b.kind;
+            a.deposit = c->p.deposit;
+            a.cmr = b.cmr;
+            a.external_bz = c->p.external_bz;
+            a.n = b.n;
+            a.diag = c->d_diag + s * kDiagPerSpecies;
+            a.energy_mode = b.energy_mode;
+            HIP_TRY(launch_push<real>(c->stream, c->g, f, jc, species_of<real>(b), a));
+        }
+        HIP_TRY(hipEventRecord(c->ev[3 * l + 1], c->stream));
+        // compute_energy's field part sees the fields before fadvance (src/main.cpp:2327)
+        HIP_TRY(launch_field_energy<real>(c->stream, c->g, f, c->d_diag + QK_MAX_SPECIES * kDiagPerSpecies));
+        // fadvance (src/advance.cpp:13-18): B half, E, B half, interpolate_be
+        HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f));
+        HIP_TRY(launch_maxwell_e<real>(c->stream, c->p.solver, c->g, c->coef, f, jc, jn));
+        HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f));
+        HIP_TRY(launch_interpolate_be<real>(c->stream, c->g, f));
+        HIP_TRY(hipEventRecord(c->ev[3 * l + 2], c->stream));
+        c->jcur ^= 1;
+        if (diag)
+            HIP_TRY(hipMemcpyAsync(hd + (size_t)l * kDiagLen, c->d_diag, kDiagLen * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float t_part = 0, t_field = 0, t_tot = 0;
+    for (int l = 0; l < nsteps; ++l) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, c->ev[3 * l], c->ev[3 * l + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, c->ev[3 * l + 1], c->ev[3 * l + 2]));
+        t_part += a;
+        t_field += b;
+    }
+    HIP_TRY(hipEventElapsedTime(&t_tot, c->ev[0], c->ev[3 * (nsteps - 1) + 2]));
+    if (diag)
+        for (int l = 0; l < nsteps; ++l) unpack_diag(c, hd + (size_t)l * kDiagLen, diag[l]);
+    c->ms_total = t_tot;
+    c->ms_part = t_part;
+    c->ms_field = t_field;
+    c->have_step = true;
+    return QK_OK;
+}
+
+template <typename T>
+void aos_to_soa(const double *aos, int64_t n, T *a, T *b, T *c)
+{
+    for (int64_t m = 0; m < n; ++m) {
+        a[m] = (T)aos[3 * m];
+        b[m] = (T)aos[3 * m + 1];
+        c[m] = (T)aos[3 * m + 2];
+    }
+}
+
+int upload3(qk_ctx *c, const double *aos, void *d0, void *d1, void *d2)
+{
+    const int64_t n = c->g.ncell;
+    std::vector<char> buf((size_t)n * c->rsize * 3);
+    if (c->rsize == 4) {
+        float *h = (float *)buf.data();
+        aos_to_soa<float>(aos, n, h, h + n, h + 2 * n);
+    } else {
+        double *h = (double *)buf.data();
+        aos_to_soa<double>(aos, n, h, h + n, h + 2 * n);
+    }
+    const size_t bytes = (size_t)n * c->rsize;
+    HIP_TRY(hipMemcpy(d0, buf.data(), bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d1, buf.data() + bytes, bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d2, buf.data() + 2 * bytes, bytes, hipMemcpyHostToDevice));
+    return QK_OK;
+}
+
+int download3(qk_ctx *c, double *aos, const void *d0, const void *d1, const void *d2)
+{
+    const int64_t n = c->g.ncell;
+    const size_t bytes = (size_t)n * c->rsize;
+    std::vector<char> buf(bytes * 3);
+    HIP_TRY(hipMemcpy(buf.data(), d0, bytes, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(buf.data() + bytes, d1, bytes, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(buf.data() + 2 * bytes, d2, bytes, hipMemcpyDeviceToHost));
+    for (int64_t m = 0; m < n; ++m)
+        for (int a = 0; a < 3; ++a)
+            aos[3 * m + a] = c->rsize == 4 ? (double)((const float *)buf.data())[a * n + m]
+                                           : ((const double *)buf.data())[a * n + m];
+    return QK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *qk_last_error(void) { return g_err.c_str(); }
+
+int qk_create(const qk_params *params, qk_ctx **out)
+{
+    if (!params || !out) return fail(QK_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const qk_params &p = *params;
+    if (p.nx < 5 || p.ny < 5 || p.nz < 5) return fail(QK_ERR_INVALID, "grid must be at least 5 cells per axis");
+    if (p.nx > 2048 || p.ny > 1024 || p.nz > 1024)
+        return fail(QK_ERR_INVALID, "grid exceeds 2048 x 1024 x 1024 (packed cell index)");
+    if (!(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0)) return fail(QK_ERR_INVALID, "dx, dy, dz, dt must be > 0");
+    if (p.solver < 0 || p.solver > 2) return fail(QK_ERR_INVALID, "unknown solver");
+    if (p.pusher < 0 || p.pusher > 1) return fail(QK_ERR_INVALID, "unknown pusher");
+    if (p.precision != QK_FP32 && p.precision != QK_FP64) return fail(QK_ERR_INVALID, "unknown precision");
+    if (p.deposit != QK_DEPOSIT_TILED && p.deposit != QK_DEPOSIT_GLOBAL) return fail(QK_ERR_INVALID, "unknown deposit");
+    if (p.n_ion < 0 || p.n_ion > QK_MAX_ION) return fail(QK_ERR_INVALID, "n_ion out of range");
+    if (p.qed) return fail(QK_ERR_NOT_BUILT, "QED is not available in this build");
+    // the reference's deposit splits a path at no more than one face per axis
+    if (p.dt >= p.dx || p.dt >= p.dy || p.dt >= p.dz)
+        return fail(QK_ERR_INVALID, "dt must be below dx, dy and dz (one face crossing per axis and step)");
+
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(QK_ERR_HIP, "no HIP device available");
+    if (p.device < 0 || p.device >= ndev) return fail(QK_ERR_INVALID, "device ordinal out of range");
+    e = hipSetDevice(p.device);
+    if (e != hipSuccess) return fail(QK_ERR_HIP, hipGetErrorString(e));
+
+    qk_ctx *c = new qk_ctx;
+    c->p = p;
+    if (c->p.norm_n == 0) c->p.norm_n = 1;
+    if (c->p.norm_ef == 0) c->p.norm_ef = 1;
+    if (c->p.norm_ep == 0) c->p.norm_ep = 1;
+    c->g.nx = p.nx;
+    c->g.ny = p.ny;
+    c->g.nz = p.nz;
+    c->g.sy = p.nz;
+    c->g.sx = p.ny * p.nz;
+    c->g.ncell = p.nx * p.ny * p.nz;
+    c->g.dx = p.dx;
+    c->g.dy = p.dy;
+    c->g.dz = p.dz;
+    c->g.dt = p.dt;
+    c->rsize = p.precision == QK_FP64 ? 8 : 4;
+    // solver coefficients (src/maxwell.cpp:24-33, 238-249)
+    for (int a = 0; a < 3; ++a) {
+        c->coef.a[a] = 0;
+        c->coef.b[a] = 1;
+    }
+    const double d[3] = {p.dx, p.dy, p.dz};
+    if (p.solver == QK_SOLVER_NDFX) {
+        const double kappa = 1.25 * (std::sqrt(3.0) - 1) * 0.5 / std::sqrt(3.0);
+        for (int a = 0; a < 3; ++a) {
+            c->coef.b[a] = 1 - kappa * p.dx * p.dx / (d[a] * d[a]);
+            c->coef.a[a] = 0.5 * (1 - c->coef.b[a]);
+        }
+        c->coef.b[0] = 1 - kappa;
+        c->coef.a[0] = 0.5 * (1 - c->coef.b[0]);
+    } else if (p.solver == QK_SOLVER_FIVE_POINT) {
+        const double PI = 3.141592653589793;
+        for (int a = 0; a < 3; ++a) {
+            c->coef.a[a] = 0.25 * (1 - d[a] / p.dt * std::sin(0.5 * PI * std::fmin(1.0, p.dt / d[a])));
+            c->coef.b[a] = 1 - 2 * c->coef.a[a];
+        }
+    }
+    auto bail = [&](int code) {
+        qk_destroy(c);
+        return code;
+    };
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(fail(QK_ERR_HIP, hipGetErrorString(e)));
+    const size_t fbytes = ((size_t)c->g.ncell * c->rsize + 255) & ~(size_t)255;
+    if ((e = hipMalloc(&c->fmem, fbytes * 15)) != hipSuccess)
+        return bail(fail(QK_ERR_HIP, std::string("field allocation: ") + hipGetErrorString(e)));
+    if ((e = hipMemset(c->fmem, 0, fbytes * 15)) != hipSuccess) return bail(fail(QK_ERR_HIP, hipGetErrorString(e)));
+    for (int a = 0; a < 15; ++a) c->F[a] = (char *)c->fmem + a * fbytes;
+    if ((e = hipMalloc(&c->d_diag, kDiagLen * sizeof(double))) != hipSuccess)
+        return bail(fail(QK_ERR_HIP, hipGetErrorString(e)));
+    *out = c;
+    return QK_OK;
+}
+
+int qk_destroy(qk_ctx *c)
+{
+    if (!c) return QK_OK;
+    // release everything even if one call fails; report the first failure
+    hipError_t first = hipSuccess;
+    auto note = [&](hipError_t e) {
+        if (first == hipSuccess && e != hipSuccess) first = e;
+    };
+    if (c->stream) note(hipStreamSynchronize(c->stream));
+    for (auto &b : c->sp)
+        if (b.mem) note(hipFree(b.mem));
+    if (c->fmem) note(hipFree(c->fmem));
+    if (c->d_diag) note(hipFree(c->d_diag));
+    if (c->h_diag) note(hipHostFree(c->h_diag));
+    for (auto e : c->ev) note(hipEventDestroy(e));
+    if (c->stream) note(hipStreamDestroy(c->stream));
+    delete c;
+    if (first != hipSuccess) return fail(QK_ERR_HIP, std::string("qk_destroy: ") + hipGetErrorString(first));
+    return QK_OK;
+}
+
+int qk_set_fields(qk_ctx *c, const double *E, const double *B)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (E) {
+        int rc = upload3(c, E, c->F[0], c->F[1], c->F[2]);
+        if (rc) return rc;
+    }
+    if (B) {
+        int rc = upload3(c, B, c->F[3], c->F[4], c->F[5]);
+        if (rc) return rc;
+    }
+    return QK_OK;
+}
+
+int qk_get_fields(qk_ctx *c, double *E, double *B, double *J, double *Be)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int rc;
+    if (E && (rc = download3(c, E, c->F[0], c->F[1], c->F[2]))) return rc;
+    if (B && (rc = download3(c, B, c->F[3], c->F[4], c->F[5]))) return rc;
+    if (Be && (rc = download3(c, Be, c->F[6], c->F[7], c->F[8]))) return rc;
+    if (J) {
+        const int w = c->have_step ? (c->jcur ^ 1) : c->jcur;   // buffer of the last step
+        if ((rc = download3(c, J, c->F[9 + 3 * w], c->F[10 + 3 * w], c->F[11 + 3 * w]))) return rc;
+    }
+    return QK_OK;
+}
+
+int qk_add_particles(qk_ctx *c, int s, int64_t n, const double *x, const double *y, const double *z, const double *ux,
+                     const double *uy, const double *uz, const double *q, const int32_t *ids, int64_t *kept)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (!species_valid(c, s)) return fail(QK_ERR_INVALID, "invalid species");
+    if (n < 0 || (n > 0 && (!x || !y || !z || !ux || !uy || !uz || !q))) return fail(QK_ERR_INVALID, "bad particle arrays");
+    // place(): keep particles in is_inside cells (src/plist.cpp:22-31)
+    std::vector<int64_t> keep;
+    keep.reserve((size_t)n);
+    for (int64_t m = 0; m < n; ++m) {
+        const int i = (int)x[m], j = (int)y[m], k = (int)z[m];
+        if (!(x[m] >= 0 && y[m] >= 0 && z[m] >= 0)) continue;
+        if (i > 0 && i < c->p.nx - 2 && j > 0 && j < c->p.ny - 2 && k > 0 && k < c->p.nz - 2) keep.push_back(m);
+    }
+    const int64_t nk = (int64_t)keep.size();
+    int rc = ensure_species(c, s, nk);
+    if (rc) return rc;
+    SpeciesBuf &b = c->sp[s];
+    const size_t r = c->rsize;
+    std::vector<int> cell((size_t)nk), id((size_t)nk);
+    std::vector<char> buf((size_t)nk * r * 8);
+    for (int64_t t = 0; t < nk; ++t) {
+        const int64_t m = keep[(size_t)t];
+        const int i = (int)x[m], j = (int)y[m], k = (int)z[m];
+        cell[(size_t)t] = pack_cell(i, j, k);
+        id[(size_t)t] = ids ? ids[m] : 0;
+        const double v[8] = {x[m] - i, y[m] - j, z[m] - k, ux[m], uy[m], uz[m], q[m], 0.0};
+        for (int a = 0; a < 8; ++a) {
+            if (r == 4) {
+                float fv = (float)v[a];
+                if (a < 3 && fv >= 1.0f) fv = 0.99999994f;   // offsets stay below 1
+                ((float *)buf.data())[(size_t)a * nk + t] = fv;
+            } else {
+                ((double *)buf.data())[(size_t)a * nk + t] = v[a];
+            }
+        }
+    }
+    if (nk > 0) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(b.cell + b.n, cell.data(), (size_t)nk * 4, hipMemcpyHostToDevice));
+        void *dst[8] = {b.ox, b.oy, b.oz, b.ux, b.uy, b.uz, b.q, b.chi};
+        for (int a = 0; a < 8; ++a)
+            if (dst[a])
+                HIP_TRY(hipMemcpy((char *)dst[a] + (size_t)b.n * r, buf.data() + (size_t)a * nk * r, (size_t)nk * r,
+                                  hipMemcpyHostToDevice));
+        if (b.id) HIP_TRY(hipMemcpy(b.id + b.n, id.data(), (size_t)nk * 4, hipMemcpyHostToDevice));
+        b.n += nk;
+    }
+    if (kept) *kept = nk;
+    return QK_OK;
+}
+
+int qk_load_uniform_plasma(qk_ctx *c, int s, int ppc, double density, double sigma, uint64_t seed, int ilo, int ihi,
+                           int64_t *added)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (!species_valid(c, s) || s == QK_PHOTON) return fail(QK_ERR_INVALID, "invalid species for a plasma");
+    if (ppc <= 0) return fail(QK_ERR_INVALID, "ppc must be positive");
+    if (ilo < 1) ilo = 1;
+    if (ihi > c->p.nx - 2) ihi = c->p.nx - 2;
+    const int64_t total = ihi > ilo ? (int64_t)(ihi - ilo) * (c->p.ny - 3) * (c->p.nz - 3) * ppc : 0;
+    int rc = ensure_species(c, s, total);
+    if (rc) return rc;
+    SpeciesBuf &b = c->sp[s];
+    const double q = (b.cmr < 0 ? -1.0 : 1.0) * density / ppc;   // init.cpp:413-414
+    if (total > 0) {
+        if (c->rsize == 4)
+            HIP_TRY(launch_load_uniform<float>(c->stream, c->g, species_of<float>(b), b.n, ppc, q, sigma, seed, ilo, ihi));
+        else
+            HIP_TRY(launch_load_uniform<double>(c->stream, c->g, species_of<double>(b), b.n, ppc, q, sigma, seed, ilo, ihi));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        b.n += total;
+    }
+    if (added) *added = total;
+    return QK_OK;
+}
+
+int qk_init_laser_cos(qk_ctx *c, const double L[16])
+{
+    if (!c || !L) return fail(QK_ERR_INVALID, "null argument");
+    hipError_t e = c->rsize == 4 ? launch_laser_cos<float>(c->stream, c->g, fields_of<float>(c), L)
+                                 : launch_laser_cos<double>(c->stream, c->g, fields_of<double>(c), L);
+    if (e == hipErrorInvalidValue) return fail(QK_ERR_INVALID, "laser direction undefined (x1 = y0 = z0 = 0)");
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+int qk_init_boundaries(qk_ctx *c)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (c->rsize == 4) {
+        HIP_TRY(launch_init_boundaries<float>(c->stream, c->p.solver, c->g, fields_of<float>(c)));
+        HIP_TRY(launch_interpolate_be<float>(c->stream, c->g, fields_of<float>(c)));
+    } else {
+        HIP_TRY(launch_init_boundaries<double>(c->stream, c->p.solver, c->g, fields_of<double>(c)));
+        HIP_TRY(launch_interpolate_be<double>(c->stream, c->g, fields_of<double>(c)));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+int qk_step(qk_ctx *c, int nsteps, qk_diag *diag)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (nsteps < 0) return fail(QK_ERR_INVALID, "negative step count");
+    if (nsteps == 0) return QK_OK;
+    return c->rsize == 4 ? step_impl<float>(c, nsteps, diag) : step_impl<double>(c, nsteps, diag);
+}
+
+int qk_sync(qk_ctx *c)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+int qk_particle_count(qk_ctx *c, int s, int64_t *n)
+{
+    if (!c || !n) return fail(QK_ERR_INVALID, "null argument");
+    if (!species_valid(c, s)) return fail(QK_ERR_INVALID, "invalid species");
+    SpeciesBuf &b = c->sp[s];
+    *n = 0;
+    if (b.n == 0) return QK_OK;
+    std::vector<int> cell((size_t)b.n);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(cell.data(), b.cell, (size_t)b.n * 4, hipMemcpyDeviceToHost));
+    int64_t live = 0;
+    for (int v : cell) live += v >= 0;
+    *n = live;
+    return QK_OK;
+}
+
+int qk_get_particles(qk_ctx *c, int s, int64_t cap, double *rec, int32_t *ids, int64_t *n_out)
+{
+    if (!c || !n_out) return fail(QK_ERR_INVALID, "null argument");
+    if (!species_valid(c, s)) return fail(QK_ERR_INVALID, "invalid species");
+    SpeciesBuf &b = c->sp[s];
+    *n_out = 0;
+    if (b.n == 0) return QK_OK;
+    const size_t r = c->rsize;
+    const int64_t n = b.n;
+    std::vector<int> cell((size_t)n), id((size_t)n);
+    std::vector<char> buf((size_t)n * r * 8, 0);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(cell.data(), b.cell, (size_t)n * 4, hipMemcpyDeviceToHost));
+    void *src[8] = {b.ox, b.oy, b.oz, b.ux, b.uy, b.uz, b.q, b.chi};
+    for (int a = 0; a < 8; ++a)
+        if (src[a]) HIP_TRY(hipMemcpy(buf.data() + (size_t)a * n * r, src[a], (size_t)n * r, hipMemcpyDeviceToHost));
+    if (b.id) HIP_TRY(hipMemcpy(id.data(), b.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+    auto get = [&](int a, int64_t m) {
+        return r == 4 ? (double)((const float *)buf.data())[(size_t)a * n + m] : ((const double *)buf.data())[(size_t)a * n + m];
+    };
+    int64_t o = 0;
+    for (int64_t m = 0; m < n; ++m) {
+        const int cc = cell[(size_t)m];
+        if (cc < 0) continue;
+        if (o >= cap) return fail(QK_ERR_CAPACITY, "output buffer too small");
+        double *w = rec + 10 * o;
+        w[0] = cell_i(cc) + get(0, m);
+        w[1] = cell_j(cc) + get(1, m);
+        w[2] = cell_k(cc) + get(2, m);
+        w[3] = get(3, m);
+        w[4] = get(4, m);
+        w[5] = get(5, m);
+        const double u2 = w[3] * w[3] + w[4] * w[4] + w[5] * w[5];
+        w[6] = b.kind == 1 ? std::sqrt(u2) : std::sqrt(1 + u2);
+        w[7] = get(6, m);
+        w[8] = b.cmr;
+        w[9] = get(7, m);
+        if (ids) ids[o] = b.id ? id[(size_t)m] : 0;
+        ++o;
+    }
+    *n_out = o;
+    return QK_OK;
+}
+
+int qk_compute_rho(qk_ctx *c, double *rho)
+{
+    if (!c || !rho) return fail(QK_ERR_INVALID, "null argument");
+    const size_t bytes = (size_t)c->g.ncell * 3 * sizeof(double);
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&d, bytes));
+    hipError_t e = hipMemsetAsync(d, 0, bytes, c->stream);
+    const int slots[3] = {QK_ELECTRON, QK_POSITRON, QK_PHOTON};
+    for (int t = 0; t < 3 && e == hipSuccess; ++t) {
+        SpeciesBuf &b = c->sp[slots[t]];
+        if (!b.active || b.n == 0) continue;
+        e = c->rsize == 4 ? launch_compute_rho<float>(c->stream, c->g, species_of<float>(b), b.n, d, t)
+                          : launch_compute_rho<double>(c->stream, c->g, species_of<double>(b), b.n, d, t);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(rho, d, bytes, hipMemcpyDeviceToHost);
+    const hipError_t ef = hipFree(d);
+    HIP_TRY(e);
+    HIP_TRY(ef);
+    return QK_OK;
+}
+
+int qk_last_timing(qk_ctx *c, double *ms_total, double *ms_particles, double *ms_fields)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (ms_total) *ms_total = c->ms_total;
+    if (ms_particles) *ms_particles = c->ms_part;
+    if (ms_fields) *ms_fields = c->ms_field;
+    return QK_OK;
+}
+
+int qk_get_stream(qk_ctx *c, void **stream)
+{
+    if (!c || !stream) return fail(QK_ERR_INVALID, "null argument");
+    *stream = (void *)c->stream;
+    return QK_OK;
+}
+
+}  // extern "C"

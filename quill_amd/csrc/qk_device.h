@@ -1,0 +1,282 @@
+// qk_device.h -- device-side building blocks of the PIC step for gfx950:
+// grid geometry, field views, gather, pushers, the charge-conserving segment
+// deposit and the counter-based random generator.  Included by the kernel
+// translation units only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qk {
+
+// ---------------------------------------------------------------------------
+// geometry
+
+struct Grid {
+    int nx, ny, nz;
+    int sx;            // stride of i: ny * nz
+    int sy;            // stride of j: nz
+    int ncell;
+    double dx, dy, dz, dt;
+};
+
+// packed cell coordinate of a particle: i in bits 20..30, j in 10..19, k in
+// 0..9; a negative value marks a free slot
+__host__ __device__ inline int pack_cell(int i, int j, int k) { return (i << 20) | (j << 10) | k; }
+__host__ __device__ inline int cell_i(int c) { return c >> 20; }
+__host__ __device__ inline int cell_j(int c) { return (c >> 10) & 1023; }
+__host__ __device__ inline int cell_k(int c) { return c & 1023; }
+
+// is_inside (src/plist.cpp:42-48)
+__device__ inline bool is_inside(const Grid &g, int i, int j, int k)
+{
+    return i > 0 && i < g.nx - 2 && j > 0 && j < g.ny - 2 && k > 0 && k < g.nz - 2;
+}
+
+template <typename real>
+struct Fields {
+    real *ex, *ey, *ez;
+    real *bx, *by, *bz;
+    real *bex, *bey, *bez;
+};
+
+template <typename real>
+struct Current {
+    real *jx, *jy, *jz;
+};
+
+// one particle species, structure of arrays
+template <typename real>
+struct Species {
+    int *cell;
+    real *ox, *oy, *oz;    // offsets inside the cell, [0, 1)
+    real *ux, *uy, *uz;    // u = p / m c
+    real *q;               // macro charge (n_cr units, signed)
+    real *chi;             // QED parameter (may be null when QED is off)
+    int *id;               // optional particle id
+};
+
+// ---------------------------------------------------------------------------
+// gather (src/deposition.cpp:431-471): each component is bilinear in the two
+// directions transverse to it, at the lower node along its own direction
+
+template <typename real>
+__device__ inline void gather3(const real *__restrict__ fx, const real *__restrict__ fy,
+                               const real *__restrict__ fz, int base, int sx, int sy, real xa, real ya,
+                               real za, real out[3])
+{
+    const real xb = real(1) - xa, yb = real(1) - ya, zb = real(1) - za;
+    out[0] = fx[base] * yb * zb + fx[base + sy] * ya * zb + fx[base + 1] * yb * za + fx[base + sy + 1] * ya * za;
+    out[1] = fy[base] * xb * zb + fy[base + sx] * xa * zb + fy[base + 1] * xb * za + fy[base + sx + 1] * xa * za;
+    out[2] = fz[base] * xb * yb + fz[base + sx] * xa * yb + fz[base + sy] * xb * ya + fz[base + sx + sy] * xa * ya;
+}
+
+// ---------------------------------------------------------------------------
+// pushers (src/pusher.cpp)
+
+// Vay 2008 (pusher.cpp:5-42); g enters as gamma at n-1/2 and leaves as gamma at n+1/2
+template <typename real>
+__device__ inline void push_vay(real cmr_dt, const real ef[3], const real bf[3], real u[3], real &g)
+{
+    const real e0 = ef[0] * cmr_dt, e1 = ef[1] * cmr_dt, e2 = ef[2] * cmr_dt;
+    const real h = real(0.5) * cmr_dt;
+    const real b0 = bf[0] * h, b1 = bf[1] * h, b2 = bf[2] * h;
+    const real ig = real(1) / g;
+    const real w0 = u[0] + e0 + (u[1] * b2 - u[2] * b1) * ig;
+    const real w1 = u[1] + e1 + (u[2] * b0 - u[0] * b2) * ig;
+    const real w2 = u[2] + e2 + (u[0] * b1 - u[1] * b0) * ig;
+    const real bb = b0 * b0 + b1 * b1 + b2 * b2;
+    const real bw = b0 * w0 + b1 * w1 + b2 * w2;
+    const real s = real(1) + w0 * w0 + w1 * w1 + w2 * w2 - bb;
+    g = sqrt(real(0.5) * (s + sqrt(s * s + real(4) * (bb + bw * bw))));
+    const real ig2 = real(1) / (g * g);
+    const real den = real(1) / (real(1) + bb * ig2);
+    const real bwg = bw * ig2;
+    const real ign = real(1) / g;
+    u[0] = (w0 + (w1 * b2 - w2 * b1) * ign + b0 * bwg) * den;
+    u[1] = (w1 + (w2 * b0 - w0 * b2) * ign + b1 * bwg) * den;
+    u[2] = (w2 + (w0 * b1 - w1 * b0) * ign + b2 * bwg) * den;
+}
+
+// Boris (pusher.cpp:44-97)
+template <typename real>
+__device__ inline void push_boris(real cmr_dt, const real ef[3], const real bf[3], real u[3], real &g)
+{
+    const real h = real(0.5) * cmr_dt;
+    const real e0 = ef[0] * h, e1 = ef[1] * h, e2 = ef[2] * h;
+    real m0 = u[0] + e0, m1 = u[1] + e1, m2 = u[2] + e2;
+    const real gm = sqrt(real(1) + m0 * m0 + m1 * m1 + m2 * m2);
+    const real hg = h / gm;
+    const real t0 = bf[0] * hg, t1 = bf[1] * hg, t2 = bf[2] * hg;
+    const real w0 = m0 + (m1 * t2 - m2 * t1);
+    const real w1 = m1 + (m2 * t0 - m0 * t2);
+    const real w2 = m2 + (m0 * t1 - m1 * t0);
+    const real f = real(2) / (real(1) + t0 * t0 + t1 * t1 + t2 * t2);
+    m0 += (w1 * t2 - w2 * t1) * f;
+    m1 += (w2 * t0 - w0 * t2) * f;
+    m2 += (w0 * t1 - w1 * t0) * f;
+    u[0] = m0 + e0;
+    u[1] = m1 + e1;
+    u[2] = m2 + e2;
+    g = sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+}
+
+template <int PUSHER, typename real>
+__device__ inline void push(real cmr_dt, const real ef[3], const real bf[3], real u[3], real &g)
+{
+    if (PUSHER == 1) push_boris(cmr_dt, ef, bf, u, g);
+    else push_vay(cmr_dt, ef, bf, u, g);
+}
+
+// ---------------------------------------------------------------------------
+// charge-conserving deposit of one straight segment (simple_jdep,
+// src/deposition.cpp:13-39).  p is the segment start relative to the cell
+// whose lower corner is w (so xa = p.x - w.x), d the segment.  The Acc policy
+// receives (component, di, dj, dk, value) with offsets relative to w.
+
+template <typename real, class Acc>
+__device__ inline void deposit_segment(Acc &acc, real q, real px, real py, real pz, real d0, real d1, real d2)
+{
+    const real xa = px, xb = real(1) - xa;
+    const real ya = py, yb = real(1) - ya;
+    const real za = pz, zb = real(1) - za;
+    const real third = real(1) / real(3);
+    const real qx = d0 * q, qy = d1 * q, qz = d2 * q;
+    const real hyz = real(0.5) * d1, hzz = real(0.5) * d2, hxx = real(0.5) * d0;
+    const real yz3 = d1 * d2 * third, xz3 = d0 * d2 * third, xy3 = d0 * d1 * third;
+    acc(0, 0, 1, 1, qx * (ya * za + hzz * ya + hyz * za + yz3));
+    acc(0, 0, 0, 1, qx * (yb * za + hzz * yb - hyz * za - yz3));
+    acc(0, 0, 1, 0, qx * (ya * zb + hyz * zb - hzz * ya - yz3));
+    acc(0, 0, 0, 0, qx * (yb * zb - hzz * yb - hyz * zb + yz3));
+    acc(1, 1, 0, 1, qy * (xa * za + hxx * za + hzz * xa + xz3));
+    acc(1, 0, 0, 1, qy * (xb * za + hzz * xb - hxx * za - xz3));
+    acc(1, 1, 0, 0, qy * (xa * zb + hxx * zb - hzz * xa - xz3));
+    acc(1, 0, 0, 0, qy * (xb * zb - hxx * zb - hzz * xb + xz3));
+    acc(2, 1, 1, 0, qz * (xa * ya + hxx * ya + hyz * xa + xy3));
+    acc(2, 0, 1, 0, qz * (xb * ya + hyz * xb - hxx * ya - xy3));
+    acc(2, 1, 0, 0, qz * (xa * yb + hxx * yb - hyz * xa - xy3));
+    acc(2, 0, 0, 0, qz * (xb * yb - hxx * yb - hyz * xb + xy3));
+}
+
+// Split the straight path d, starting at p (relative to the start cell, so
+// 0 <= p < 1 on each axis), at every crossed cell face and deposit each piece
+// (jdeposition, src/deposition.cpp:41-356).  At most one face per axis is
+// crossed (|d| < 1).  With one or two crossings the pieces run through the
+// start cell, the cell with the first-crossed index switched, and the end
+// cell; ties go to the axis the reference tests first.  With three crossings
+// the fractions are sorted (regulate, src/low-level.cpp:316-366) and each
+// piece's cell is the floor of its midpoint.  The Acc policy gets a
+// set_cell(wi, wj, wk) call (offsets of the piece's cell from the start
+// cell) before each piece.  On return p holds the end point, advanced piece
+// by piece as the reference advances the particle.
+template <typename real, class Acc>
+__device__ inline void deposit_path(Acc &acc, real q, real p[3], const real d[3])
+{
+    int c2[3];
+    c2[0] = (int)floor(p[0] + d[0]);
+    c2[1] = (int)floor(p[1] + d[1]);
+    c2[2] = (int)floor(p[2] + d[2]);
+    const int cx = c2[0] != 0, cy = c2[1] != 0, cz = c2[2] != 0;
+    const int nc = cx + cy + cz;
+    int w[3] = {0, 0, 0};
+    real s[3];
+    auto piece = [&](real f0, real f1, real f2) {
+        acc.set_cell(w[0], w[1], w[2]);
+        deposit_segment<real>(acc, q, p[0] - real(w[0]), p[1] - real(w[1]), p[2] - real(w[2]), f0, f1, f2);
+        p[0] += f0;
+        p[1] += f1;
+        p[2] += f2;
+    };
+    if (nc == 0) {
+        piece(d[0], d[1], d[2]);
+    } else if (nc == 1) {
+        const int a = cx ? 0 : (cy ? 1 : 2);
+        const real face = real(c2[a] > 0 ? c2[a] : 0);
+        const real t = face - p[a];
+        for (int m = 0; m < 3; ++m) s[m] = (m == a) ? t : d[m] * t / d[a];
+        piece(s[0], s[1], s[2]);
+        w[a] = c2[a];
+        piece(d[0] - s[0], d[1] - s[1], d[2] - s[2]);
+    } else if (nc == 2) {
+        int a0, a1;
+        if (!cx) { a0 = 1; a1 = 2; }
+        else if (!cy) { a0 = 0; a1 = 2; }
+        else { a0 = 1; a1 = 0; }
+        const real t0 = (real(c2[a0] > 0 ? c2[a0] : 0) - p[a0]) / d[a0];
+        const real t1 = (real(c2[a1] > 0 ? c2[a1] : 0) - p[a1]) / d[a1];
+        int first = a0, second = a1;
+        real tf = t0, ts = t1;
+        if (t0 > t1) { first = a1; second = a0; tf = t1; ts = t0; }
+        piece(d[0] * tf, d[1] * tf, d[2] * tf);
+        w[first] = c2[first];
+        const real tm = ts - tf;
+        piece(d[0] * tm, d[1] * tm, d[2] * tm);
+        w[second] = c2[second];
+        const real te = real(1) - ts;
+        piece(d[0] * te, d[1] * te, d[2] * te);
+    } else {
+        real t[3];
+        for (int m = 0; m < 3; ++m) t[m] = (real(c2[m] > 0 ? c2[m] : 0) - p[m]) / d[m];
+        // descending order as regulate() returns it
+        real v0, v1, v2;
+        if (t[0] > t[1]) {
+            if (t[1] > t[2]) { v0 = t[0]; v1 = t[1]; v2 = t[2]; }
+            else if (t[0] > t[2]) { v0 = t[0]; v1 = t[2]; v2 = t[1]; }
+            else { v0 = t[2]; v1 = t[0]; v2 = t[1]; }
+        } else {
+            if (t[2] > t[1]) { v0 = t[2]; v1 = t[1]; v2 = t[0]; }
+            else if (t[0] > t[2]) { v0 = t[1]; v1 = t[0]; v2 = t[2]; }
+            else { v0 = t[1]; v1 = t[2]; v2 = t[0]; }
+        }
+        const real fr[4] = {v2, v1 - v2, v0 - v1, real(1) - v0};
+        for (int n = 0; n < 4; ++n) {
+            s[0] = d[0] * fr[n];
+            s[1] = d[1] * fr[n];
+            s[2] = d[2] * fr[n];
+            if (n > 0) {
+                w[0] = (int)floor(p[0] + real(0.5) * s[0]);
+                w[1] = (int)floor(p[1] + real(0.5) * s[1]);
+                w[2] = (int)floor(p[2] + real(0.5) * s[2]);
+            }
+            piece(s[0], s[1], s[2]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11)
+
+struct u32x4 {
+    uint32_t v[4];
+};
+
+__host__ __device__ inline u32x4 philox(u32x4 c, uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.v[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.v[2];
+        u32x4 n;
+        n.v[0] = (uint32_t)(p1 >> 32) ^ c.v[1] ^ k0;
+        n.v[1] = (uint32_t)p1;
+        n.v[2] = (uint32_t)(p0 >> 32) ^ c.v[3] ^ k1;
+        n.v[3] = (uint32_t)p0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+__host__ __device__ inline double u24(uint32_t r) { return (double)(r >> 8) * (1.0 / 16777216.0); }
+__host__ __device__ inline double u24_open0(uint32_t r) { return (double)((r >> 8) + 1u) * (1.0 / 16777216.0); }
+
+// ---------------------------------------------------------------------------
+// wave64 reductions
+
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+}  // namespace qk

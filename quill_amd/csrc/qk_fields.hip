@@ -1,0 +1,451 @@
+// qk_fields.hip -- field kernels of the step on gfx950: the three Maxwell
+// solvers (FDTD, NDFX, five-point) with the reference's boundary-plane sets,
+// B interpolation to E positions, boundary zeroing, the laser initial
+// condition and the FP64 field-energy reduction.
+//
+// Fields are structure-of-arrays, one array per component, [nx][ny][nz] with
+// k fastest; a thread owns one cell and consecutive lanes walk k, so every
+// stencil load of a wavefront is a coalesced run.
+#include "qk_kernels.h"
+
+namespace qk {
+
+namespace {
+
+constexpr int kBX = 64, kBY = 4;   // workgroup: 64 lanes along k, 4 rows along j
+
+inline dim3 cell_grid(const Grid &g) { return dim3((g.nz + kBX - 1) / kBX, (g.ny + kBY - 1) / kBY, g.nx); }
+
+__device__ inline bool in(int v, int lo, int hi) { return v >= lo && v <= hi; }
+
+// which cells get component C of B updated
+// (FDTD src/maxwell.cpp:445-484, NDFX :251-290, five-point :35-85)
+template <int S, int C>
+__device__ inline bool b_updates(const Grid &g, int i, int j, int k)
+{
+    const int c[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+    if (S == 2) {   // each index in [1, n-3] or at n-1, at most one at n-1
+        int top = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (c[a] == n[a] - 1) ++top;
+            else if (!in(c[a], 1, n[a] - 3)) return false;
+        }
+        return top <= 1;
+    }
+    const int lo = (S == 0) ? 0 : 1;   // lowest plane updated
+    int nlow = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a == C) {
+            if (!in(c[a], lo + 1, n[a] - 2)) return false;
+        } else {
+            if (!in(c[a], lo, n[a] - 2)) return false;
+            nlow += (c[a] == lo);
+        }
+    }
+    return nlow <= 1;
+}
+
+// which cells get component C of E updated
+// (FDTD src/maxwell.cpp:486-522, NDFX :292-328, five-point :87-125)
+template <int S, int C>
+__device__ inline bool e_updates(const Grid &g, int i, int j, int k)
+{
+    const int c[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+    const int lo = (S == 0) ? 0 : 1;
+    const int top = (S == 2) ? 3 : 2;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int first = (a == C) ? lo : lo + 1;
+        if (!in(c[a], first, n[a] - top)) return false;
+    }
+    return true;
+}
+
+template <typename real>
+struct Coef {
+    real a[3], b[3];
+    real dtd[3];   // dt / d_alpha
+    real d[3];     // d_alpha
+};
+
+template <int A>
+__device__ inline int stride(const Grid &g)
+{
+    return A == 0 ? g.sx : (A == 1 ? g.sy : 1);
+}
+
+// zero outside the grid; used by the five-point planes at n-1 (src/maxwell.cpp:52-84)
+template <typename real>
+__device__ inline real rd0(const Grid &g, const real *F, int i, int j, int k)
+{
+    if ((unsigned)i >= (unsigned)g.nx || (unsigned)j >= (unsigned)g.ny || (unsigned)k >= (unsigned)g.nz) return real(0);
+    return F[i * g.sx + j * g.sy + k];
+}
+
+// Difference of component C (stored in F) along axis A at cell (i,j,k);
+// FWD: F[+1] - F[0] (B update), else F[0] - F[-1] (E update).
+//   FDTD:  plain difference
+//   NDFX:  smoothed along the component's own axis C with (b_C, a_C)
+//   five:  b_A (F[+1]-F[0]) + a_A (F[+2]-F[+1]+F[0]-F[-1]), zero outside
+template <int S, int C, int A, bool FWD, typename real>
+__device__ inline real diff(const Grid &g, const Coef<real> &cf, const real *__restrict__ F, int i, int j, int k)
+{
+    const int ua = (A == 0), va = (A == 1), wa = (A == 2);
+    const int li = FWD ? i : i - ua, lj = FWD ? j : j - va, lk = FWD ? k : k - wa;
+    const int lo = li * g.sx + lj * g.sy + lk;
+    const int sA = stride<A>(g);
+    const int hi = lo + sA;
+    if (S == 0) {
+        return F[hi] - F[lo];
+    } else if (S == 1) {
+        const int sC = stride<C>(g);
+        return cf.b[C] * (F[hi] - F[lo]) + cf.a[C] * ((F[hi + sC] - F[lo + sC]) + (F[hi - sC] - F[lo - sC]));
+    } else {
+        const real fm = rd0(g, F, li - ua, lj - va, lk - wa);
+        const real f0 = rd0(g, F, li, lj, lk);
+        const real f1 = rd0(g, F, li + ua, lj + va, lk + wa);
+        const real f2 = rd0(g, F, li + 2 * ua, lj + 2 * va, lk + 2 * wa);
+        return cf.b[A] * (f1 - f0) + cf.a[A] * (f2 - f1 + f0 - fm);
+    }
+}
+
+// half step of B (maxwell_solver::advance_b, src/maxwell.cpp:10-22)
+template <int S, typename real>
+__global__ void __launch_bounds__(kBX * kBY) k_maxwell_b(Grid g, Coef<real> cf, const real *__restrict__ ex,
+                                                         const real *__restrict__ ey, const real *__restrict__ ez,
+                                                         real *__restrict__ bx, real *__restrict__ by,
+                                                         real *__restrict__ bz)
+{
+    const int k = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const int id = i * g.sx + j * g.sy + k;
+    const real h = real(0.5);
+    if (b_updates<S, 0>(g, i, j, k))
+        bx[id] += h * (-cf.dtd[1] * diff<S, 2, 1, true>(g, cf, ez, i, j, k) + cf.dtd[2] * diff<S, 1, 2, true>(g, cf, ey, i, j, k));
+    if (b_updates<S, 1>(g, i, j, k))
+        by[id] += h * (cf.dtd[0] * diff<S, 2, 0, true>(g, cf, ez, i, j, k) - cf.dtd[2] * diff<S, 0, 2, true>(g, cf, ex, i, j, k));
+    if (b_updates<S, 2>(g, i, j, k))
+        bz[id] += h * (cf.dtd[1] * diff<S, 0, 1, true>(g, cf, ex, i, j, k) - cf.dtd[0] * diff<S, 1, 0, true>(g, cf, ey, i, j, k));
+}
+
+// E step with the current of this step; also clears the other current
+// buffer, which the next step deposits into (replaces the J reset of
+// src/advance.cpp:112-123 without a separate pass)
+template <int S, typename real>
+__global__ void __launch_bounds__(kBX * kBY) k_maxwell_e(Grid g, Coef<real> cf, real *__restrict__ ex, real *__restrict__ ey,
+                                                         real *__restrict__ ez, const real *__restrict__ bx,
+                                                         const real *__restrict__ by, const real *__restrict__ bz,
+                                                         const real *__restrict__ jx, const real *__restrict__ jy,
+                                                         const real *__restrict__ jz, real *__restrict__ njx,
+                                                         real *__restrict__ njy, real *__restrict__ njz)
+{
+    const int k = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const int id = i * g.sx + j * g.sy + k;
+    if (e_updates<S, 0>(g, i, j, k))
+        ex[id] += cf.dtd[1] * diff<S, 2, 1, false>(g, cf, bz, i, j, k) - cf.dtd[2] * diff<S, 1, 2, false>(g, cf, by, i, j, k) - jx[id] * cf.d[0];
+    if (e_updates<S, 1>(g, i, j, k))
+        ey[id] += -cf.dtd[0] * diff<S, 2, 0, false>(g, cf, bz, i, j, k) + cf.dtd[2] * diff<S, 0, 2, false>(g, cf, bx, i, j, k) - jy[id] * cf.d[1];
+    if (e_updates<S, 2>(g, i, j, k))
+        ez[id] += cf.dtd[0] * diff<S, 1, 0, false>(g, cf, by, i, j, k) - cf.dtd[1] * diff<S, 0, 1, false>(g, cf, bx, i, j, k) - jz[id] * cf.d[2];
+    njx[id] = real(0);
+    njy[id] = real(0);
+    njz[id] = real(0);
+}
+
+// Be = B averaged to E positions (src/advance.cpp:20-83): each component is
+// the mean over a 2x2x2 stencil; on a boundary plane the half that leaves the
+// grid is dropped (4-point mean); cells on two or more boundary planes keep
+// their value.
+template <int C, typename real>
+__device__ inline real be_component(const Grid &g, const real *__restrict__ F, int i, int j, int k)
+{
+    // stencil offsets along x, y, z for component C
+    const int ox = (C == 0) ? 1 : -1, oy = (C == 1) ? 1 : -1, oz = (C == 2) ? 1 : -1;
+    const int i2 = i + ox, j2 = j + oy, k2 = k + oz;
+    const bool ux = (unsigned)i2 < (unsigned)g.nx, uy = (unsigned)j2 < (unsigned)g.ny, uz = (unsigned)k2 < (unsigned)g.nz;
+    real s = real(0);
+    int cnt = 0;
+    const int ia[2] = {i, i2}, ja[2] = {j, j2}, ka[2] = {k, k2};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !ux) continue;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            if (b == 1 && !uy) continue;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (c == 1 && !uz) continue;
+                s += F[ia[a] * g.sx + ja[b] * g.sy + ka[c]];
+                ++cnt;
+            }
+        }
+    }
+    return (cnt == 8 ? real(0.125) : real(0.25)) * s;
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kBX * kBY) k_interpolate_be(Grid g, const real *__restrict__ bx,
+                                                              const real *__restrict__ by, const real *__restrict__ bz,
+                                                              real *__restrict__ bex, real *__restrict__ bey,
+                                                              real *__restrict__ bez)
+{
+    const int k = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const int nb = (i == 0 || i == g.nx - 1) + (j == 0 || j == g.ny - 1) + (k == 0 || k == g.nz - 1);
+    if (nb >= 2) return;
+    const int id = i * g.sx + j * g.sy + k;
+    bex[id] = be_component<0>(g, bx, i, j, k);
+    bey[id] = be_component<1>(g, by, i, j, k);
+    bez[id] = be_component<2>(g, bz, i, j, k);
+}
+
+// init_boundaries: zero components on boundary planes, transverse indices in
+// [0, n-2] (FDTD src/maxwell.cpp:532-597, NDFX :338-438, five-point :135-236)
+template <typename real>
+__global__ void __launch_bounds__(kBX * kBY) k_init_boundaries(Grid g, int solver, Fields<real> f)
+{
+    const int k = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const int c[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+    int em = 0, bm = 0;
+    for (int a = 0; a < 3; ++a) {
+        const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+        if (c[b1] > n[b1] - 2 || c[b2] > n[b2] - 2) continue;
+        const int normal = 1 << a, tang = 7 & ~normal;
+        const int p = c[a];
+        if (solver == 0) {          // FDTD
+            if (p == 0) { em |= tang; bm |= normal; }
+            if (p == n[a] - 1) { em |= 7; bm |= 7; }
+        } else if (solver == 1) {   // NDFX
+            if (p == 0 || p == n[a] - 1) { em |= 7; bm |= 7; }
+            if (p == 1) { em |= tang; bm |= normal; }
+        } else {                    // five-point
+            if (p == 0 || p == n[a] - 1) { em |= 7; bm |= 7; }
+            if (p == 1) em |= tang;
+            if (p == n[a] - 2) bm |= tang;
+        }
+    }
+    const int id = i * g.sx + j * g.sy + k;
+    if (em & 1) f.ex[id] = real(0);
+    if (em & 2) f.ey[id] = real(0);
+    if (em & 4) f.ez[id] = real(0);
+    if (bm & 1) f.bx[id] = real(0);
+    if (bm & 2) f.by[id] = real(0);
+    if (bm & 4) f.bz[id] = real(0);
+}
+
+// sum of E^2 + B^2 over all cells in FP64 (compute_energy, src/fout.cpp:102)
+template <typename real>
+__global__ void __launch_bounds__(256) k_field_energy(int n, Fields<real> f, double *acc)
+{
+    __shared__ double part[4];
+    double s = 0;
+    for (int id = blockIdx.x * 256 + threadIdx.x; id < n; id += gridDim.x * 256) {
+        const double a = f.ex[id], b = f.ey[id], c = f.ez[id];
+        const double d = f.bx[id], e = f.by[id], h = f.bz[id];
+        s += a * a + b * b + c * c + d * d + e * e + h * h;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
+}
+
+// Laser initial condition f_init_cos with the cos envelope (src/init.cpp:6-210),
+// evaluated in FP64 and stored in the field precision.  L: a0y a0z xsigma
+// ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt
+struct LaserFrame {
+    double a0y, a0z, xs, ys, zs, x0, phase, xt, yt, zt, y12, z12;
+    double r0[3], ey[3], ez[3];
+    double y0, z0;
+    int b_sign, append;
+};
+
+template <typename real>
+__global__ void __launch_bounds__(kBX * kBY) k_laser_cos(Grid g, Fields<real> f, LaserFrame L)
+{
+    const int k = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const double PI = 3.141592653589793;
+    const double X = i * g.dx - L.x0 - L.xt, Y = j * g.dy - L.y12 - L.y0 - L.yt, Z = k * g.dz - L.z12 - L.z0 - L.zt;
+    const double x = L.r0[0] * X + L.r0[1] * Y + L.r0[2] * Z;
+    const double y = L.ey[0] * X + L.ey[1] * Y + L.ey[2] * Z;
+    const double z = L.ez[0] * X + L.ez[1] * Y + L.ez[2] * Z;
+    const int id = i * g.sx + j * g.sy + k;
+    real *E[3] = {f.ex, f.ey, f.ez}, *B[3] = {f.bx, f.by, f.bz};
+    if (x < L.xs && x > -L.xs && y < L.ys && y > -L.ys && z < L.zs && z > -L.zs) {
+        const double xi = x + L.phase;
+        const double cx = cos(PI * x / 2 / L.xs), cy = cos(PI * y / 2 / L.ys), cz = cos(PI * z / 2 / L.zs);
+        const double sx = sin(PI * x / 2 / L.xs), sy = sin(PI * y / 2 / L.ys), sz = sin(PI * z / 2 / L.zs);
+        const double sxi = sin(xi), cxi = cos(xi);
+        const double tr = cy * cy * cz * cz;
+        const double fy = L.a0y * tr * (cxi * cx * cx - PI / L.xs * sxi * cx * sx);
+        const double fz = L.a0z * tr * (sxi * cx * cx + PI / L.xs * cxi * cx * sx);
+        const double fx = L.a0y * sxi * cx * cx * cz * cz * PI / L.ys * cy * sy - L.a0z * cxi * cx * cx * cy * cy * PI / L.zs * cz * sz;
+        double gx = L.a0z * cxi * cx * cx * PI / L.ys * cz * cz * cy * sy + L.a0y * sxi * cx * cx * cy * cy * PI / L.zs * cz * sz;
+        double gy = -fz, gz = fy;
+        if (!L.b_sign) { gx = -gx; gy = -gy; gz = -gz; }
+        for (int m = 0; m < 3; ++m) {
+            const double ev = fx * L.r0[m] + fy * L.ey[m] + fz * L.ez[m];
+            const double bv = gx * L.r0[m] + gy * L.ey[m] + gz * L.ez[m];
+            if (L.append) {
+                E[m][id] = (real)((double)E[m][id] + ev);
+                B[m][id] = (real)((double)B[m][id] + bv);
+            } else {
+                E[m][id] = (real)ev;
+                B[m][id] = (real)bv;
+            }
+        }
+    } else if (!L.append) {
+        for (int m = 0; m < 3; ++m) { E[m][id] = real(0); B[m][id] = real(0); }
+    }
+}
+
+template <typename real>
+Coef<real> make_coef(const Grid &g, const SolverCoef &c)
+{
+    Coef<real> r;
+    const double d[3] = {g.dx, g.dy, g.dz};
+    for (int a = 0; a < 3; ++a) {
+        r.a[a] = (real)c.a[a];
+        r.b[a] = (real)c.b[a];
+        r.dtd[a] = (real)(g.dt / d[a]);
+        r.d[a] = (real)d[a];
+    }
+    return r;
+}
+
+__global__ void k_zero(double *p, size_t n)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
+}  // namespace
+
+hipError_t launch_zero(hipStream_t st, double *p, size_t n)
+{
+    if (n == 0) return hipSuccess;
+    k_zero<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_maxwell_b(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f)
+{
+    const Coef<real> cf = make_coef<real>(g, c);
+    const dim3 grid = cell_grid(g), block(kBX, kBY);
+    if (solver == 0) k_maxwell_b<0, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz);
+    else if (solver == 1) k_maxwell_b<1, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz);
+    else k_maxwell_b<2, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_maxwell_e(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f,
+                            const Current<real> &j, const Current<real> &jn)
+{
+    const Coef<real> cf = make_coef<real>(g, c);
+    const dim3 grid = cell_grid(g), block(kBX, kBY);
+#define QK_E_ARGS g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz, j.jx, j.jy, j.jz, jn.jx, jn.jy, jn.jz
+    if (solver == 0) k_maxwell_e<0, real><<<grid, block, 0, st>>>(QK_E_ARGS);
+    else if (solver == 1) k_maxwell_e<1, real><<<grid, block, 0, st>>>(QK_E_ARGS);
+    else k_maxwell_e<2, real><<<grid, block, 0, st>>>(QK_E_ARGS);
+#undef QK_E_ARGS
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_interpolate_be(hipStream_t st, const Grid &g, const Fields<real> &f)
+{
+    k_interpolate_be<real><<<cell_grid(g), dim3(kBX, kBY), 0, st>>>(g, f.bx, f.by, f.bz, f.bex, f.bey, f.bez);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_init_boundaries(hipStream_t st, int solver, const Grid &g, const Fields<real> &f)
+{
+    k_init_boundaries<real><<<cell_grid(g), dim3(kBX, kBY), 0, st>>>(g, solver, f);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_field_energy(hipStream_t st, const Grid &g, const Fields<real> &f, double *acc)
+{
+    int blocks = (g.ncell + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    k_field_energy<real><<<blocks, 256, 0, st>>>(g.ncell, f, acc);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f, const double *P)
+{
+    // propagation frame (src/init.cpp:32-93)
+    const double PI = 3.141592653589793;
+    LaserFrame L;
+    L.a0y = P[0];
+    L.a0z = P[1];
+    L.xs = P[2] * 2 * sqrt(2 * PI) / 3;
+    L.ys = P[3] * 2 * sqrt(2 * PI) / 3;
+    L.zs = P[4] * 2 * sqrt(2 * PI) / 3;
+    L.x0 = P[5];
+    L.b_sign = P[6] != 0;
+    const double x1 = P[7];
+    L.phase = P[8];
+    L.y0 = P[9];
+    L.z0 = P[10];
+    L.append = P[11] != 0;
+    const double phi = P[12];
+    L.xt = P[13];
+    L.yt = P[14];
+    L.zt = P[15];
+    L.y12 = 0.5 * g.ny * g.dy;
+    L.z12 = 0.5 * g.nz * g.dz;
+    const double rn = sqrt(x1 * x1 + L.y0 * L.y0 + L.z0 * L.z0);
+    if (!(rn > 0)) return hipErrorInvalidValue;
+    L.r0[0] = -x1 / rn;
+    L.r0[1] = -L.y0 / rn;
+    L.r0[2] = -L.z0 / rn;
+    const double t = sqrt(L.r0[0] * L.r0[0] + L.r0[1] * L.r0[1]);
+    if (t != 0) { L.ey[0] = -L.r0[1] / t; L.ey[1] = L.r0[0] / t; L.ey[2] = 0; }
+    else { L.ey[0] = 0; L.ey[1] = 1; L.ey[2] = 0; }
+    L.ez[0] = L.r0[1] * L.ey[2] - L.r0[2] * L.ey[1];
+    L.ez[1] = L.r0[2] * L.ey[0] - L.r0[0] * L.ey[2];
+    L.ez[2] = L.r0[0] * L.ey[1] - L.r0[1] * L.ey[0];
+    if (phi != 0) {
+        const double c = cos(phi), s = sin(phi);
+        double y[3], z[3];
+        for (int m = 0; m < 3; ++m) { y[m] = L.ey[m]; z[m] = L.ez[m]; }
+        for (int m = 0; m < 3; ++m) { L.ey[m] = y[m] * c + z[m] * s; L.ez[m] = z[m] * c - y[m] * s; }
+    }
+    k_laser_cos<real><<<cell_grid(g), dim3(kBX, kBY), 0, st>>>(g, f, L);
+    return hipGetLastError();
+}
+
+#define QK_INST(real)                                                                                            \
+    template hipError_t launch_maxwell_b<real>(hipStream_t, int, const Grid &, const SolverCoef &, const Fields<real> &); \
+    template hipError_t launch_maxwell_e<real>(hipStream_t, int, const Grid &, const SolverCoef &, const Fields<real> &, \
+                                               const Current<real> &, const Current<real> &);                   \
+    template hipError_t launch_interpolate_be<real>(hipStream_t, const Grid &, const Fields<real> &);           \
+    template hipError_t launch_init_boundaries<real>(hipStream_t, int, const Grid &, const Fields<real> &);     \
+    template hipError_t launch_field_energy<real>(hipStream_t, const Grid &, const Fields<real> &, double *);   \
+    template hipError_t launch_laser_cos<real>(hipStream_t, const Grid &, const Fields<real> &, const double *);
+QK_INST(float)
+QK_INST(double)
+#undef QK_INST
+
+}  // namespace qk

@@ -1,0 +1,292 @@
+// qk_particles.hip -- particle kernels of the step on gfx950: the fused
+// gather + push + charge-conserving deposit + re-binning kernel (the particle
+// loop of src/advance.cpp:90-387), photon transport, the device plasma loader
+// and the node-charge deposit used for the Gauss-law check.
+#include "qk_kernels.h"
+
+namespace qk {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// every contribution is a global atomic (the plain method)
+template <typename real>
+struct GlobalAcc {
+    real *jx, *jy, *jz;
+    int sx, sy;
+    int base;   // linear index of the particle's start cell
+    int cur;    // linear index of the current piece's cell
+    __device__ inline void set_cell(int wi, int wj, int wk) { cur = base + wi * sx + wj * sy + wk; }
+    __device__ inline void operator()(int comp, int di, int dj, int dk, real v)
+    {
+        const int id = cur + di * sx + dj * sy + dk;
+        real *a = comp == 0 ? jx : (comp == 1 ? jy : jz);
+        unsafeAtomicAdd(a + id, v);
+    }
+};
+
+__device__ inline void block_accumulate(double *dst, double v0, double v1, double v2)
+{
+    __shared__ double part[3][kBlock / 64];
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    v2 = wave_sum(v2);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][w] = v0;
+        part[1][w] = v1;
+        part[2][w] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int m = 0; m < kBlock / 64; ++m) {
+            s0 += part[0][m];
+            s1 += part[1][m];
+            s2 += part[2][m];
+        }
+        if (s2 != 0) {
+            atomicAdd(dst + 0, s0);
+            atomicAdd(dst + 1, s1);
+            atomicAdd(dst + 2, s2);
+        }
+    }
+}
+
+// Charged particles (electrons, positrons, ions) with QED off: gather E and
+// Be at the particle, add external Bz, push (Vay or Boris), deposit the
+// current of the straight path u dt / gamma split at cell faces, move, and
+// drop the particle if its new cell is not is_inside (place(),
+// src/plist.cpp:7-40).  Diagnostics of compute_N / compute_energy
+// (src/fout.cpp:41-138) are accumulated from the post-push state.
+template <typename real, int PUSHER>
+__global__ void __launch_bounds__(kBlock) k_push_global(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
+                                                        int64_t n, real cmr_dt, real ext_bz, real inv_dx, real inv_dy,
+                                                        real inv_dz, real dt, double *diag, int energy_mode,
+                                                        double inv_cmr)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double dq = 0, de = 0, dn = 0;
+    if (p < n) {
+        const int c = s.cell[p];
+        if (c >= 0) {
+            const int i = cell_i(c), j = cell_j(c), k = cell_k(c);
+            real pos[3] = {s.ox[p], s.oy[p], s.oz[p]};
+            real u[3] = {s.ux[p], s.uy[p], s.uz[p]};
+            const real q = s.q[p];
+            const int base = i * g.sx + j * g.sy + k;
+            real e[3], b[3];
+            gather3(f.ex, f.ey, f.ez, base, g.sx, g.sy, pos[0], pos[1], pos[2], e);
+            gather3(f.bex, f.bey, f.bez, base, g.sx, g.sy, pos[0], pos[1], pos[2], b);
+            b[2] += ext_bz;
+            real gam = sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            push<PUSHER>(cmr_dt, e, b, u, gam);
+            // displacement in cells (containers.cpp:58-67, advance.cpp:281-284)
+            const real a = dt / gam;
+            const real d[3] = {u[0] * a * inv_dx, u[1] * a * inv_dy, u[2] * a * inv_dz};
+            // |d| < dt/dx < 1 for any finite momentum; a non-finite one (corrupt
+            // input) would index outside the stencil, so the particle is dropped
+            const bool finite = fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1);
+            if (!finite) s.cell[p] = -1;
+            if (finite) {
+            GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
+            deposit_path<real>(acc, q, pos, d);
+            // re-bin: new cell and offset
+            int ci[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const real fl = floor(pos[m]);
+                ci[m] = (int)fl;
+                pos[m] -= fl;
+                if (pos[m] >= real(1)) {   // rounding of a tiny negative position
+                    pos[m] = real(0);
+                    ci[m] += 1;
+                }
+            }
+            const int ni = i + ci[0], nj = j + ci[1], nk = k + ci[2];
+            s.ux[p] = u[0];
+            s.uy[p] = u[1];
+            s.uz[p] = u[2];
+            if (is_inside(g, ni, nj, nk)) {
+                s.cell[p] = pack_cell(ni, nj, nk);
+                s.ox[p] = pos[0];
+                s.oy[p] = pos[1];
+                s.oz[p] = pos[2];
+                dq = (double)q;
+                dn = 1;
+                const double gg = (double)gam;
+                de = energy_mode == 2 ? (gg - 1) * (double)q * inv_cmr : (gg - 1) * (double)q;
+            } else {
+                s.cell[p] = -1;
+            }
+            }   // finite
+        }
+    }
+    block_accumulate(diag, dq, de, dn);
+}
+
+// Photons with QED off move on straight lines (src/advance.cpp:173-183) and
+// carry no current.
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_photon_move(Grid g, Species<real> s, int64_t n, real inv_dx, real inv_dy,
+                                                        real inv_dz, real dt, double *diag)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double dq = 0, de = 0, dn = 0;
+    if (p < n) {
+        const int c = s.cell[p];
+        if (c >= 0) {
+            const real u[3] = {s.ux[p], s.uy[p], s.uz[p]};
+            const real gam = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            const real a = dt / gam;
+            real pos[3] = {s.ox[p] + u[0] * a * inv_dx, s.oy[p] + u[1] * a * inv_dy, s.oz[p] + u[2] * a * inv_dz};
+            int ci[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const real fl = floor(pos[m]);
+                ci[m] = (int)fl;
+                pos[m] -= fl;
+                if (pos[m] >= real(1)) {
+                    pos[m] = real(0);
+                    ci[m] += 1;
+                }
+            }
+            const int ni = cell_i(c) + ci[0], nj = cell_j(c) + ci[1], nk = cell_k(c) + ci[2];
+            const bool finite = fabs(u[0] * a) < real(1e30) && fabs(u[1] * a) < real(1e30) && fabs(u[2] * a) < real(1e30);
+            if (finite && is_inside(g, ni, nj, nk)) {
+                s.cell[p] = pack_cell(ni, nj, nk);
+                s.ox[p] = pos[0];
+                s.oy[p] = pos[1];
+                s.oz[p] = pos[2];
+                const double q = s.q[p];
+                dq = q;
+                dn = 1;
+                de = (double)gam * q;
+            } else {
+                s.cell[p] = -1;
+            }
+        }
+    }
+    block_accumulate(diag, dq, de, dn);
+}
+
+// Uniform thermal plasma: ppc particles per is_inside cell with ilo <= i < ihi,
+// Philox keyed by (seed, cell * ppc + slot), 24-bit uniform offsets and
+// Box-Muller normal momenta; written in cell order after slot `first`.
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_load_uniform(Grid g, Species<real> s, int64_t first, int ppc, double q,
+                                                         double sigma, uint64_t seed, int ilo, int ihi, int64_t total)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const int slot = (int)(t % ppc);
+    int64_t c = t / ppc;
+    const int nk = g.nz - 3, nj = g.ny - 3;
+    const int k = 1 + (int)(c % nk);
+    c /= nk;
+    const int j = 1 + (int)(c % nj);
+    const int i = ilo + (int)(c / nj);
+    const uint64_t cell = ((uint64_t)i * g.ny + j) * g.nz + k;
+    const uint64_t gid = cell * (uint64_t)ppc + slot;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const u32x4 a = philox(u32x4{{(uint32_t)gid, (uint32_t)(gid >> 32), 0u, 0x51ADu}}, k0, k1);
+    const u32x4 b = philox(u32x4{{(uint32_t)gid, (uint32_t)(gid >> 32), 1u, 0x51ADu}}, k0, k1);
+    const double two_pi = 6.283185307179586;
+    const double r1 = sqrt(-2.0 * log(u24_open0(a.v[3])));
+    const double t1 = two_pi * u24(b.v[0]);
+    const double r2 = sqrt(-2.0 * log(u24_open0(b.v[1])));
+    const double t2 = two_pi * u24(b.v[2]);
+    const int64_t o = first + t;
+    s.cell[o] = pack_cell(i, j, k);
+    s.ox[o] = (real)u24(a.v[0]);
+    s.oy[o] = (real)u24(a.v[1]);
+    s.oz[o] = (real)u24(a.v[2]);
+    s.ux[o] = (real)(sigma * r1 * cos(t1));
+    s.uy[o] = (real)(sigma * r1 * sin(t1));
+    s.uz[o] = (real)(sigma * r2 * cos(t2));
+    s.q[o] = (real)q;
+    if (s.chi) s.chi[o] = real(0);
+    if (s.id) s.id[o] = (int)gid;
+}
+
+// node charge with trilinear weights (rhodeposition, src/deposition.cpp:358-429)
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_compute_rho(Grid g, Species<real> s, int64_t n, double *rho, int slot)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const int c = s.cell[p];
+    if (c < 0) return;
+    const int i = cell_i(c), j = cell_j(c), k = cell_k(c);
+    const double xa = s.ox[p], ya = s.oy[p], za = s.oz[p];
+    const double xb = 1 - xa, yb = 1 - ya, zb = 1 - za;
+    const double q = s.q[p];
+    const int b = i * g.sx + j * g.sy + k;
+    double *r = rho + slot;
+    unsafeAtomicAdd(r + 3 * (int64_t)b, q * xb * yb * zb);
+    unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sx), q * xa * yb * zb);
+    unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sy), q * xb * ya * zb);
+    unsafeAtomicAdd(r + 3 * (int64_t)(b + 1), q * xb * yb * za);
+    unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sx + g.sy), q * xa * ya * zb);
+    unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sx + 1), q * xa * yb * za);
+    unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sy + 1), q * xb * ya * za);
+    unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sx + g.sy + 1), q * xa * ya * za);
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+template <typename real>
+hipError_t launch_push(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
+                       const Species<real> &s, const PushArgs &a)
+{
+    if (a.n <= 0) return hipSuccess;
+    const real dt = (real)g.dt;
+    const real idx = (real)(1.0 / g.dx), idy = (real)(1.0 / g.dy), idz = (real)(1.0 / g.dz);
+    if (a.kind == 1) {
+        k_photon_move<real><<<blocks_for(a.n), kBlock, 0, st>>>(g, s, a.n, idx, idy, idz, dt, a.diag);
+        return hipGetLastError();
+    }
+    const real cmr_dt = (real)(a.cmr * g.dt);
+    const double inv_cmr = a.cmr != 0 ? 1.0 / a.cmr : 0.0;
+    if (a.pusher == 1)
+        k_push_global<real, 1><<<blocks_for(a.n), kBlock, 0, st>>>(g, f, j, s, a.n, cmr_dt, (real)a.external_bz, idx,
+                                                                   idy, idz, dt, a.diag, a.energy_mode, inv_cmr);
+    else
+        k_push_global<real, 0><<<blocks_for(a.n), kBlock, 0, st>>>(g, f, j, s, a.n, cmr_dt, (real)a.external_bz, idx,
+                                                                   idy, idz, dt, a.diag, a.energy_mode, inv_cmr);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_load_uniform(hipStream_t st, const Grid &g, const Species<real> &s, int64_t first, int ppc,
+                               double q, double sigma, uint64_t seed, int ilo, int ihi)
+{
+    const int64_t total = (int64_t)(ihi - ilo) * (g.ny - 3) * (g.nz - 3) * ppc;
+    if (total <= 0) return hipSuccess;
+    k_load_uniform<real><<<blocks_for(total), kBlock, 0, st>>>(g, s, first, ppc, q, sigma, seed, ilo, ihi, total);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_compute_rho(hipStream_t st, const Grid &g, const Species<real> &s, int64_t n, double *rho, int slot)
+{
+    if (n <= 0) return hipSuccess;
+    k_compute_rho<real><<<blocks_for(n), kBlock, 0, st>>>(g, s, n, rho, slot);
+    return hipGetLastError();
+}
+
+#define QK_INST(real)                                                                                            \
+    template hipError_t launch_push<real>(hipStream_t, const Grid &, const Fields<real> &, const Current<real> &,  \
+                                          const Species<real> &, const PushArgs &);                              \
+    template hipError_t launch_load_uniform<real>(hipStream_t, const Grid &, const Species<real> &, int64_t, int, \
+                                                  double, double, uint64_t, int, int);                           \
+    template hipError_t launch_compute_rho<real>(hipStream_t, const Grid &, const Species<real> &, int64_t,       \
+                                                 double *, int);
+QK_INST(float)
+QK_INST(double)
+#undef QK_INST
+
+}  // namespace qk
