@@ -96,11 +96,12 @@ def timed_run(w, precision, deposit, steps, warmup):
     r.sync()
     t1 = time.perf_counter()
     ms_total, ms_part, ms_field = r.timing()
+    fallback = r.fallback_fraction()
     # particles pushed in step l are those alive before it
     pushed = n_start + sum(d["n_e"] for d in diag[:-1])
     r.close()
     return dict(seconds=t1 - t0, pushed=pushed, n_start=n_start, n_loaded=n0, ms_events=ms_total,
-                ms_particles=ms_part, ms_fields=ms_field, energy_f_last=diag[-1]["energy_f"])
+                ms_particles=ms_part, ms_fields=ms_field, energy_f_last=diag[-1]["energy_f"], fallback=fallback)
 
 
 def field_error(w, steps):
@@ -219,6 +220,8 @@ def main():
         "cell_updates_per_s": ncell * K / main_run["seconds"],
         "ms_particles_per_step": push_ms,
         "ms_fields_per_step": main_run["ms_fields"] / K,
+        "ms_sort_and_launch_per_step": ms_step - main_run["ms_events"] / K,
+        "deposit_fallback_fraction": main_run["fallback"],
     }
     if not args.no_compare:
         k64 = min(args.fp64_steps, K)

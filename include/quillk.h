@@ -138,6 +138,10 @@ int qk_compute_rho(qk_ctx *ctx, double *rho);
 /* timing of the last qk_step call, measured with HIP events on the context's
  * stream: total milliseconds and the part spent in the particle kernels */
 int qk_last_timing(qk_ctx *ctx, double *ms_total, double *ms_particles, double *ms_fields);
+/* fraction of the particle pushes of the last qk_step call whose current went
+ * through global atomics because the particle had left its tile's LDS window
+ * since the last sort (0 for QK_DEPOSIT_GLOBAL, where every push does) */
+int qk_last_fallback(qk_ctx *ctx, double *fraction);
 /* the context's HIP stream (hipStream_t) */
 int qk_get_stream(qk_ctx *ctx, void **stream);
 

@@ -31,7 +31,8 @@ SPECIES_NAMES = {"electron": ELECTRON, "positron": POSITRON, "photon": PHOTON}
 
 EXPORTS = ["qk_create", "qk_destroy", "qk_last_error", "qk_set_fields", "qk_get_fields", "qk_add_particles",
            "qk_load_uniform_plasma", "qk_init_laser_cos", "qk_init_boundaries", "qk_step", "qk_sync",
-           "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_get_stream"]
+           "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_last_fallback",
+           "qk_get_stream"]
 
 
 class QuillError(RuntimeError):
@@ -91,6 +92,7 @@ def load_library(path=LIB_PATH):
     L.qk_get_particles.argtypes = [vp, C.c_int, C.c_int64, dp, ip, i64p]
     L.qk_compute_rho.argtypes = [vp, dp]
     L.qk_last_timing.argtypes = [vp, dp, dp, dp]
+    L.qk_last_fallback.argtypes = [vp, dp]
     L.qk_get_stream.argtypes = [vp, C.POINTER(vp)]
     for name in EXPORTS:
         if name not in ("qk_last_error",):
@@ -266,6 +268,12 @@ class Region:
         t = [C.c_double(), C.c_double(), C.c_double()]
         _check(load_library().qk_last_timing(self._h, *[C.byref(v) for v in t]), "qk_last_timing")
         return tuple(v.value for v in t)
+
+    def fallback_fraction(self):
+        """Share of the last step() call's pushes that deposited through global atomics."""
+        v = C.c_double()
+        _check(load_library().qk_last_fallback(self._h, C.byref(v)), "qk_last_fallback")
+        return v.value
 
     def stream(self):
         s = C.c_void_p()

@@ -3,6 +3,7 @@
 // step driver in the order of src/main.cpp:2319-2332, and copies in and out.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,18 +31,35 @@ int fail(int code, const std::string &msg)
         if (e_ != hipSuccess) return fail(QK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define QK_TRY(expr)              \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != QK_OK) return rc_; \
+    } while (0)
+
+// the arrays of one particle buffer (structure of arrays)
+struct SoA {
+    void *mem = nullptr;   // one allocation holding every array
+    int *cell = nullptr;
+    void *a[8] = {};       // ox oy oz ux uy uz q chi (chi only with QED)
+    int *id = nullptr;     // only with track_ids
+};
+
 struct SpeciesBuf {
     bool active = false;
     double cmr = 0;
-    int kind = 0;         // 0 charged, 1 photon
-    int energy_mode = 0;  // 0 (g-1) q, 1 g q, 2 (g-1) q / cmr
+    int kind = 0;           // 0 charged, 1 photon
+    int energy_mode = 0;    // 0 (g-1) q, 1 g q, 2 (g-1) q / cmr
     int64_t cap = 0;
-    int64_t n = 0;        // slots in use (live or free)
-    void *mem = nullptr;  // one allocation holding every array
-    int *cell = nullptr;
-    void *ox = nullptr, *oy = nullptr, *oz = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *q = nullptr,
-         *chi = nullptr;
-    int *id = nullptr;
+    int64_t n = 0;          // slots in use (live or free)
+    int64_t n_sorted = 0;   // slots [0, n_sorted) are in tile order with a valid work list
+    SoA buf[2];             // current and sort target
+    int cur = 0;
+    int4 *chunks = nullptr; // tiled-kernel work list
+    int *d_nchunks = nullptr;
+    int max_chunks = 0;     // capacity of chunks
+    int nchunks = 0;        // host copy after the last sort
+    int steps_since_sort = 0;
 };
 
 }  // namespace
@@ -49,6 +67,7 @@ struct SpeciesBuf {
 struct qk_ctx {
     qk_params p;
     Grid g;
+    TileGeom tg;
     SolverCoef coef;
     hipStream_t stream = nullptr;
     size_t rsize = 4;               // bytes per real
@@ -57,14 +76,20 @@ struct qk_ctx {
     int jcur = 0;                   // current buffer deposited this step
     bool have_step = false;
     SpeciesBuf sp[QK_MAX_SPECIES];
-    double *d_diag = nullptr;       // [QK_MAX_SPECIES][3] + field energy
+    // shared sort buffers (allocated at the first sort)
+    int *s_counts = nullptr, *s_offsets = nullptr, *s_tile_start = nullptr, *s_chunk_n = nullptr,
+        *s_chunk_off = nullptr, *s_scratch = nullptr;
+    int *h_sortinfo = nullptr;      // pinned: [live count, chunk count]
+    double *d_diag = nullptr;       // [QK_MAX_SPECIES][kDiagPerSpecies] + field energy
     double *h_diag = nullptr;       // pinned per-step copies of the accumulators
     size_t h_diag_cap = 0;          // steps that fit in h_diag
     std::vector<hipEvent_t> ev;     // timing events
-    double ms_total = 0, ms_part = 0, ms_field = 0;
+    double ms_total = 0, ms_part = 0, ms_field = 0, fallback_frac = 0;
 };
 
 namespace {
+
+constexpr size_t kDiagLen = QK_MAX_SPECIES * kDiagPerSpecies + 1;
 
 template <typename real>
 Fields<real> fields_of(qk_ctx *c)
@@ -87,14 +112,14 @@ Current<real> current_of(qk_ctx *c, int which)
 }
 
 template <typename real>
-Species<real> species_of(SpeciesBuf &b)
+Species<real> view(const SoA &b)
 {
     Species<real> s;
     s.cell = b.cell;
-    s.ox = (real *)b.ox; s.oy = (real *)b.oy; s.oz = (real *)b.oz;
-    s.ux = (real *)b.ux; s.uy = (real *)b.uy; s.uz = (real *)b.uz;
-    s.q = (real *)b.q;
-    s.chi = (real *)b.chi;
+    s.ox = (real *)b.a[0]; s.oy = (real *)b.a[1]; s.oz = (real *)b.a[2];
+    s.ux = (real *)b.a[3]; s.uy = (real *)b.a[4]; s.uz = (real *)b.a[5];
+    s.q = (real *)b.a[6];
+    s.chi = (real *)b.a[7];
     s.id = b.id;
     return s;
 }
@@ -114,44 +139,64 @@ bool species_valid(const qk_ctx *c, int s)
     return true;
 }
 
-// (re)allocate the arrays of one species to hold cap slots, keeping n
-int reserve(qk_ctx *c, int s, int64_t cap)
+int alloc_soa(qk_ctx *c, SoA &b, int64_t cap)
 {
-    SpeciesBuf &b = c->sp[s];
-    if (cap <= b.cap) return QK_OK;
     const size_t r = c->rsize;
     const bool chi = c->p.qed != 0, ids = c->p.track_ids != 0;
-    // per slot: cell (4) + 7 reals + chi + id
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t a_int = align((size_t)cap * 4), a_real = align((size_t)cap * r);
     const size_t bytes = a_int + 7 * a_real + (chi ? a_real : 0) + (ids ? a_int : 0);
     void *mem = nullptr;
     HIP_TRY(hipMalloc(&mem, bytes));
     char *m = (char *)mem;
-    SpeciesBuf nb = b;
-    nb.mem = mem;
-    nb.cap = cap;
-    nb.cell = (int *)m; m += a_int;
-    nb.ox = m; m += a_real;
-    nb.oy = m; m += a_real;
-    nb.oz = m; m += a_real;
-    nb.ux = m; m += a_real;
-    nb.uy = m; m += a_real;
-    nb.uz = m; m += a_real;
-    nb.q = m; m += a_real;
-    nb.chi = chi ? (void *)m : nullptr; m += chi ? a_real : 0;
-    nb.id = ids ? (int *)m : nullptr;
-    if (b.mem && b.n > 0) {
-        HIP_TRY(hipMemcpyAsync(nb.cell, b.cell, (size_t)b.n * 4, hipMemcpyDeviceToDevice, c->stream));
-        void *src[8] = {b.ox, b.oy, b.oz, b.ux, b.uy, b.uz, b.q, b.chi};
-        void *dst[8] = {nb.ox, nb.oy, nb.oz, nb.ux, nb.uy, nb.uz, nb.q, nb.chi};
+    b.mem = mem;
+    b.cell = (int *)m;
+    m += a_int;
+    for (int a = 0; a < 7; ++a, m += a_real) b.a[a] = m;
+    b.a[7] = chi ? (void *)m : nullptr;
+    m += chi ? a_real : 0;
+    b.id = ids ? (int *)m : nullptr;
+    return QK_OK;
+}
+
+// (re)allocate both buffers of one species to cap slots, keeping slots [0, n)
+int reserve(qk_ctx *c, int s, int64_t cap)
+{
+    SpeciesBuf &b = c->sp[s];
+    if (cap <= b.cap) return QK_OK;
+    SoA nb[2];
+    for (int k = 0; k < 2; ++k) {
+        int rc = alloc_soa(c, nb[k], cap);
+        if (rc) {
+            for (int m = 0; m < k; ++m) (void)hipFree(nb[m].mem);
+            return rc;
+        }
+    }
+    const size_t r = c->rsize;
+    SoA &old = b.buf[b.cur];
+    if (old.mem && b.n > 0) {
+        HIP_TRY(hipMemcpyAsync(nb[0].cell, old.cell, (size_t)b.n * 4, hipMemcpyDeviceToDevice, c->stream));
         for (int a = 0; a < 8; ++a)
-            if (src[a] && dst[a]) HIP_TRY(hipMemcpyAsync(dst[a], src[a], (size_t)b.n * r, hipMemcpyDeviceToDevice, c->stream));
-        if (b.id && nb.id) HIP_TRY(hipMemcpyAsync(nb.id, b.id, (size_t)b.n * 4, hipMemcpyDeviceToDevice, c->stream));
+            if (old.a[a] && nb[0].a[a])
+                HIP_TRY(hipMemcpyAsync(nb[0].a[a], old.a[a], (size_t)b.n * r, hipMemcpyDeviceToDevice, c->stream));
+        if (old.id && nb[0].id) HIP_TRY(hipMemcpyAsync(nb[0].id, old.id, (size_t)b.n * 4, hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    if (b.mem) HIP_TRY(hipFree(b.mem));
-    b = nb;
+    for (int k = 0; k < 2; ++k)
+        if (b.buf[k].mem) HIP_TRY(hipFree(b.buf[k].mem));
+    b.buf[0] = nb[0];
+    b.buf[1] = nb[1];
+    b.cur = 0;
+    b.cap = cap;
+    // work list capacity: every tile may end in a partial chunk
+    const int64_t mc = (int64_t)c->tg.ntiles + cap / kChunk + 1;
+    if (b.chunks) HIP_TRY(hipFree(b.chunks));
+    b.chunks = nullptr;
+    HIP_TRY(hipMalloc(&b.chunks, (size_t)mc * sizeof(int4)));
+    b.max_chunks = (int)mc;
+    if (!b.d_nchunks) HIP_TRY(hipMalloc(&b.d_nchunks, sizeof(int)));
+    b.n_sorted = 0;   // the work list refers to the old buffers
+    b.nchunks = 0;
     return QK_OK;
 }
 
@@ -166,7 +211,8 @@ int ensure_species(qk_ctx *c, int s, int64_t extra)
     }
     const int64_t need = b.n + extra;
     if (need <= b.cap) return QK_OK;
-    int64_t cap = c->p.capacity > 0 ? c->p.capacity : 2 * need;
+    if (need >= ((int64_t)1 << 31)) return fail(QK_ERR_CAPACITY, "more than 2^31 particles in one species");
+    int64_t cap = c->p.capacity > 0 ? c->p.capacity : std::max<int64_t>(need + need / 4, 1024);
     if (cap < need) {
         if (c->p.capacity > 0)
             return fail(QK_ERR_CAPACITY, "particle capacity " + std::to_string(c->p.capacity) + " exceeded (need " +
@@ -174,6 +220,49 @@ int ensure_species(qk_ctx *c, int s, int64_t extra)
         cap = need;
     }
     return reserve(c, s, cap);
+}
+
+int ensure_sort_buffers(qk_ctx *c)
+{
+    if (c->s_counts) return QK_OK;
+    const size_t nk = (size_t)c->tg.nkeys + 1, nt = (size_t)c->tg.ntiles + 1;
+    HIP_TRY(hipMalloc(&c->s_counts, nk * sizeof(int)));
+    HIP_TRY(hipMalloc(&c->s_offsets, nk * sizeof(int)));
+    HIP_TRY(hipMalloc(&c->s_tile_start, nt * sizeof(int)));
+    HIP_TRY(hipMalloc(&c->s_chunk_n, nt * sizeof(int)));
+    HIP_TRY(hipMalloc(&c->s_chunk_off, nt * sizeof(int)));
+    HIP_TRY(hipMalloc(&c->s_scratch, scan_scratch_ints(std::max(nk, nt)) * sizeof(int)));
+    HIP_TRY(hipHostMalloc((void **)&c->h_sortinfo, 2 * sizeof(int)));
+    return QK_OK;
+}
+
+// sort species s by (tile, cell), compacting free slots; builds its work list
+template <typename real>
+int sort_species(qk_ctx *c, int s)
+{
+    QK_TRY(ensure_sort_buffers(c));
+    SpeciesBuf &b = c->sp[s];
+    SortBuffers sb;
+    sb.counts = c->s_counts;
+    sb.offsets = c->s_offsets;
+    sb.tile_start = c->s_tile_start;
+    sb.chunk_n = c->s_chunk_n;
+    sb.chunk_off = c->s_chunk_off;
+    sb.chunks = b.chunks;
+    sb.scratch = c->s_scratch;
+    sb.max_chunks = b.max_chunks;
+    HIP_TRY(launch_sort<real>(c->stream, c->tg, view<real>(b.buf[b.cur]), view<real>(b.buf[b.cur ^ 1]), b.n, sb));
+    HIP_TRY(hipMemcpyAsync(b.d_nchunks, c->s_chunk_off + c->tg.ntiles, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_sortinfo, c->s_tile_start + c->tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_sortinfo + 1, c->s_chunk_off + c->tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    b.cur ^= 1;
+    b.n = c->h_sortinfo[0];
+    b.n_sorted = b.n;
+    b.nchunks = c->h_sortinfo[1];
+    b.steps_since_sort = 0;
+    if (b.nchunks > b.max_chunks) return fail(QK_ERR_STATE, "tile work list overflow");
+    return QK_OK;
 }
 
 void unpack_diag(const qk_ctx *c, const double *h, qk_diag &d)
@@ -208,27 +297,28 @@ int ensure_events(qk_ctx *c, size_t n)
     return QK_OK;
 }
 
-constexpr size_t kDiagLen = QK_MAX_SPECIES * kDiagPerSpecies + 1;
-
 template <typename real>
 int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
 {
     const Fields<real> f = fields_of<real>(c);
     // three events per step: before the particle phase, between phases, after the field phase
-    int rc = ensure_events(c, 3 * (size_t)nsteps);
-    if (rc) return rc;
-    std::vector<double> hbuf;
-    double *hd = nullptr;
-    if (diag) {
-        if (c->h_diag_cap < (size_t)nsteps) {
-            if (c->h_diag) HIP_TRY(hipHostFree(c->h_diag));
-            c->h_diag = nullptr;
-            HIP_TRY(hipHostMalloc((void **)&c->h_diag, (size_t)nsteps * kDiagLen * sizeof(double)));
-            c->h_diag_cap = (size_t)nsteps;
-        }
-        hd = c->h_diag;
+    QK_TRY(ensure_events(c, 3 * (size_t)nsteps));
+    if (c->h_diag_cap < (size_t)nsteps) {
+        if (c->h_diag) HIP_TRY(hipHostFree(c->h_diag));
+        c->h_diag = nullptr;
+        HIP_TRY(hipHostMalloc((void **)&c->h_diag, (size_t)nsteps * kDiagLen * sizeof(double)));
+        c->h_diag_cap = (size_t)nsteps;
     }
+    double *hd = c->h_diag;
+    const bool tiled = c->p.deposit == QK_DEPOSIT_TILED;
     for (int l = 0; l < nsteps; ++l) {
+        // cell-tile sort: particles outside the sorted region, or the cadence
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            SpeciesBuf &b = c->sp[s];
+            if (!b.active || b.n == 0 || b.kind == 1) continue;
+            const bool due = c->p.sort_every > 0 && b.steps_since_sort >= c->p.sort_every;
+            if ((tiled && b.n_sorted < b.n) || due) QK_TRY(sort_species<real>(c, s));
+        }
         const Current<real> jc = current_of<real>(c, c->jcur), jn = current_of<real>(c, c->jcur ^ 1);
         HIP_TRY(launch_zero(c->stream, c->d_diag, kDiagLen));
         // padvance (src/advance.cpp:90-387); the boundary erase of p_boundary
@@ -241,13 +331,25 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
             PushArgs a{};
             a.pusher = c->p.pusher;
             a.kind = b.kind;
-            a.deposit = c->p.deposit;
             a.cmr = b.cmr;
             a.external_bz = c->p.external_bz;
-            a.n = b.n;
             a.diag = c->d_diag + s * kDiagPerSpecies;
             a.energy_mode = b.energy_mode;
-            HIP_TRY(launch_push<real>(c->stream, c->g, f, jc, species_of<real>(b), a));
+            const Species<real> v = view<real>(b.buf[b.cur]);
+            if (tiled && b.kind == 0) {
+                a.chunks = b.chunks;
+                a.nchunks = b.d_nchunks;
+                a.max_chunks = b.nchunks;
+                a.tg = c->tg;
+                HIP_TRY(launch_push_tiled<real>(c->stream, c->g, f, jc, v, a));
+                a.first = b.n_sorted;   // the unsorted tail, if any
+                a.n = b.n - b.n_sorted;
+            } else {
+                a.first = 0;
+                a.n = b.n;
+            }
+            HIP_TRY(launch_push_global<real>(c->stream, c->g, f, jc, v, a));
+            ++b.steps_since_sort;
         }
         HIP_TRY(hipEventRecord(c->ev[3 * l + 1], c->stream));
         // compute_energy's field part sees the fields before fadvance (src/main.cpp:2327)
@@ -259,9 +361,8 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         HIP_TRY(launch_interpolate_be<real>(c->stream, c->g, f));
         HIP_TRY(hipEventRecord(c->ev[3 * l + 2], c->stream));
         c->jcur ^= 1;
-        if (diag)
-            HIP_TRY(hipMemcpyAsync(hd + (size_t)l * kDiagLen, c->d_diag, kDiagLen * sizeof(double),
-                                   hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(hd + (size_t)l * kDiagLen, c->d_diag, kDiagLen * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     float t_part = 0, t_field = 0, t_tot = 0;
@@ -273,8 +374,16 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         t_field += b;
     }
     HIP_TRY(hipEventElapsedTime(&t_tot, c->ev[0], c->ev[3 * (nsteps - 1) + 2]));
-    if (diag)
-        for (int l = 0; l < nsteps; ++l) unpack_diag(c, hd + (size_t)l * kDiagLen, diag[l]);
+    double fb = 0, live = 0;
+    for (int l = 0; l < nsteps; ++l) {
+        const double *h = hd + (size_t)l * kDiagLen;
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            live += h[s * kDiagPerSpecies + 2];
+            fb += h[s * kDiagPerSpecies + 3];
+        }
+        if (diag) unpack_diag(c, h, diag[l]);
+    }
+    c->fallback_frac = live > 0 ? fb / live : 0;
     c->ms_total = t_tot;
     c->ms_part = t_part;
     c->ms_field = t_field;
@@ -345,6 +454,7 @@ int qk_create(const qk_params *params, qk_ctx **out)
     if (p.precision != QK_FP32 && p.precision != QK_FP64) return fail(QK_ERR_INVALID, "unknown precision");
     if (p.deposit != QK_DEPOSIT_TILED && p.deposit != QK_DEPOSIT_GLOBAL) return fail(QK_ERR_INVALID, "unknown deposit");
     if (p.n_ion < 0 || p.n_ion > QK_MAX_ION) return fail(QK_ERR_INVALID, "n_ion out of range");
+    if (p.sort_every < 0) return fail(QK_ERR_INVALID, "sort_every must be >= 0");
     if (p.qed) return fail(QK_ERR_NOT_BUILT, "QED is not available in this build");
     // the reference's deposit splits a path at no more than one face per axis
     if (p.dt >= p.dx || p.dt >= p.dy || p.dt >= p.dz)
@@ -372,6 +482,11 @@ int qk_create(const qk_params *params, qk_ctx **out)
     c->g.dy = p.dy;
     c->g.dz = p.dz;
     c->g.dt = p.dt;
+    c->tg.ntx = (p.nx + kTile - 1) / kTile;
+    c->tg.nty = (p.ny + kTile - 1) / kTile;
+    c->tg.ntz = (p.nz + kTile - 1) / kTile;
+    c->tg.ntiles = c->tg.ntx * c->tg.nty * c->tg.ntz;
+    c->tg.nkeys = c->tg.ntiles * kTileCells;
     c->rsize = p.precision == QK_FP64 ? 8 : 4;
     // solver coefficients (src/maxwell.cpp:24-33, 238-249)
     for (int a = 0; a < 3; ++a) {
@@ -420,8 +535,16 @@ int qk_destroy(qk_ctx *c)
         if (first == hipSuccess && e != hipSuccess) first = e;
     };
     if (c->stream) note(hipStreamSynchronize(c->stream));
-    for (auto &b : c->sp)
-        if (b.mem) note(hipFree(b.mem));
+    for (auto &b : c->sp) {
+        for (auto &k : b.buf)
+            if (k.mem) note(hipFree(k.mem));
+        if (b.chunks) note(hipFree(b.chunks));
+        if (b.d_nchunks) note(hipFree(b.d_nchunks));
+    }
+    int *sb[6] = {c->s_counts, c->s_offsets, c->s_tile_start, c->s_chunk_n, c->s_chunk_off, c->s_scratch};
+    for (int *p : sb)
+        if (p) note(hipFree(p));
+    if (c->h_sortinfo) note(hipHostFree(c->h_sortinfo));
     if (c->fmem) note(hipFree(c->fmem));
     if (c->d_diag) note(hipFree(c->d_diag));
     if (c->h_diag) note(hipHostFree(c->h_diag));
@@ -436,14 +559,8 @@ int qk_set_fields(qk_ctx *c, const double *E, const double *B)
 {
     if (!c) return fail(QK_ERR_INVALID, "null context");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (E) {
-        int rc = upload3(c, E, c->F[0], c->F[1], c->F[2]);
-        if (rc) return rc;
-    }
-    if (B) {
-        int rc = upload3(c, B, c->F[3], c->F[4], c->F[5]);
-        if (rc) return rc;
-    }
+    if (E) QK_TRY(upload3(c, E, c->F[0], c->F[1], c->F[2]));
+    if (B) QK_TRY(upload3(c, B, c->F[3], c->F[4], c->F[5]));
     return QK_OK;
 }
 
@@ -451,13 +568,12 @@ int qk_get_fields(qk_ctx *c, double *E, double *B, double *J, double *Be)
 {
     if (!c) return fail(QK_ERR_INVALID, "null context");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    int rc;
-    if (E && (rc = download3(c, E, c->F[0], c->F[1], c->F[2]))) return rc;
-    if (B && (rc = download3(c, B, c->F[3], c->F[4], c->F[5]))) return rc;
-    if (Be && (rc = download3(c, Be, c->F[6], c->F[7], c->F[8]))) return rc;
+    if (E) QK_TRY(download3(c, E, c->F[0], c->F[1], c->F[2]));
+    if (B) QK_TRY(download3(c, B, c->F[3], c->F[4], c->F[5]));
+    if (Be) QK_TRY(download3(c, Be, c->F[6], c->F[7], c->F[8]));
     if (J) {
         const int w = c->have_step ? (c->jcur ^ 1) : c->jcur;   // buffer of the last step
-        if ((rc = download3(c, J, c->F[9 + 3 * w], c->F[10 + 3 * w], c->F[11 + 3 * w]))) return rc;
+        QK_TRY(download3(c, J, c->F[9 + 3 * w], c->F[10 + 3 * w], c->F[11 + 3 * w]));
     }
     return QK_OK;
 }
@@ -472,14 +588,14 @@ int qk_add_particles(qk_ctx *c, int s, int64_t n, const double *x, const double 
     std::vector<int64_t> keep;
     keep.reserve((size_t)n);
     for (int64_t m = 0; m < n; ++m) {
+        if (!(x[m] >= 0 && y[m] >= 0 && z[m] >= 0 && x[m] < c->p.nx && y[m] < c->p.ny && z[m] < c->p.nz)) continue;
         const int i = (int)x[m], j = (int)y[m], k = (int)z[m];
-        if (!(x[m] >= 0 && y[m] >= 0 && z[m] >= 0)) continue;
         if (i > 0 && i < c->p.nx - 2 && j > 0 && j < c->p.ny - 2 && k > 0 && k < c->p.nz - 2) keep.push_back(m);
     }
     const int64_t nk = (int64_t)keep.size();
-    int rc = ensure_species(c, s, nk);
-    if (rc) return rc;
+    QK_TRY(ensure_species(c, s, nk));
     SpeciesBuf &b = c->sp[s];
+    SoA &dst = b.buf[b.cur];
     const size_t r = c->rsize;
     std::vector<int> cell((size_t)nk), id((size_t)nk);
     std::vector<char> buf((size_t)nk * r * 8);
@@ -501,13 +617,12 @@ int qk_add_particles(qk_ctx *c, int s, int64_t n, const double *x, const double 
     }
     if (nk > 0) {
         HIP_TRY(hipStreamSynchronize(c->stream));
-        HIP_TRY(hipMemcpy(b.cell + b.n, cell.data(), (size_t)nk * 4, hipMemcpyHostToDevice));
-        void *dst[8] = {b.ox, b.oy, b.oz, b.ux, b.uy, b.uz, b.q, b.chi};
+        HIP_TRY(hipMemcpy(dst.cell + b.n, cell.data(), (size_t)nk * 4, hipMemcpyHostToDevice));
         for (int a = 0; a < 8; ++a)
-            if (dst[a])
-                HIP_TRY(hipMemcpy((char *)dst[a] + (size_t)b.n * r, buf.data() + (size_t)a * nk * r, (size_t)nk * r,
+            if (dst.a[a])
+                HIP_TRY(hipMemcpy((char *)dst.a[a] + (size_t)b.n * r, buf.data() + (size_t)a * nk * r, (size_t)nk * r,
                                   hipMemcpyHostToDevice));
-        if (b.id) HIP_TRY(hipMemcpy(b.id + b.n, id.data(), (size_t)nk * 4, hipMemcpyHostToDevice));
+        if (dst.id) HIP_TRY(hipMemcpy(dst.id + b.n, id.data(), (size_t)nk * 4, hipMemcpyHostToDevice));
         b.n += nk;
     }
     if (kept) *kept = nk;
@@ -523,15 +638,14 @@ int qk_load_uniform_plasma(qk_ctx *c, int s, int ppc, double density, double sig
     if (ilo < 1) ilo = 1;
     if (ihi > c->p.nx - 2) ihi = c->p.nx - 2;
     const int64_t total = ihi > ilo ? (int64_t)(ihi - ilo) * (c->p.ny - 3) * (c->p.nz - 3) * ppc : 0;
-    int rc = ensure_species(c, s, total);
-    if (rc) return rc;
+    QK_TRY(ensure_species(c, s, total));
     SpeciesBuf &b = c->sp[s];
     const double q = (b.cmr < 0 ? -1.0 : 1.0) * density / ppc;   // init.cpp:413-414
     if (total > 0) {
         if (c->rsize == 4)
-            HIP_TRY(launch_load_uniform<float>(c->stream, c->g, species_of<float>(b), b.n, ppc, q, sigma, seed, ilo, ihi));
+            HIP_TRY(launch_load_uniform<float>(c->stream, c->g, view<float>(b.buf[b.cur]), b.n, ppc, q, sigma, seed, ilo, ihi));
         else
-            HIP_TRY(launch_load_uniform<double>(c->stream, c->g, species_of<double>(b), b.n, ppc, q, sigma, seed, ilo, ihi));
+            HIP_TRY(launch_load_uniform<double>(c->stream, c->g, view<double>(b.buf[b.cur]), b.n, ppc, q, sigma, seed, ilo, ihi));
         HIP_TRY(hipStreamSynchronize(c->stream));
         b.n += total;
     }
@@ -588,7 +702,7 @@ int qk_particle_count(qk_ctx *c, int s, int64_t *n)
     if (b.n == 0) return QK_OK;
     std::vector<int> cell((size_t)b.n);
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(cell.data(), b.cell, (size_t)b.n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cell.data(), b.buf[b.cur].cell, (size_t)b.n * 4, hipMemcpyDeviceToHost));
     int64_t live = 0;
     for (int v : cell) live += v >= 0;
     *n = live;
@@ -602,16 +716,17 @@ int qk_get_particles(qk_ctx *c, int s, int64_t cap, double *rec, int32_t *ids, i
     SpeciesBuf &b = c->sp[s];
     *n_out = 0;
     if (b.n == 0) return QK_OK;
+    if (!rec) return fail(QK_ERR_INVALID, "null output buffer");
+    const SoA &src = b.buf[b.cur];
     const size_t r = c->rsize;
     const int64_t n = b.n;
     std::vector<int> cell((size_t)n), id((size_t)n);
     std::vector<char> buf((size_t)n * r * 8, 0);
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(cell.data(), b.cell, (size_t)n * 4, hipMemcpyDeviceToHost));
-    void *src[8] = {b.ox, b.oy, b.oz, b.ux, b.uy, b.uz, b.q, b.chi};
+    HIP_TRY(hipMemcpy(cell.data(), src.cell, (size_t)n * 4, hipMemcpyDeviceToHost));
     for (int a = 0; a < 8; ++a)
-        if (src[a]) HIP_TRY(hipMemcpy(buf.data() + (size_t)a * n * r, src[a], (size_t)n * r, hipMemcpyDeviceToHost));
-    if (b.id) HIP_TRY(hipMemcpy(id.data(), b.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+        if (src.a[a]) HIP_TRY(hipMemcpy(buf.data() + (size_t)a * n * r, src.a[a], (size_t)n * r, hipMemcpyDeviceToHost));
+    if (src.id) HIP_TRY(hipMemcpy(id.data(), src.id, (size_t)n * 4, hipMemcpyDeviceToHost));
     auto get = [&](int a, int64_t m) {
         return r == 4 ? (double)((const float *)buf.data())[(size_t)a * n + m] : ((const double *)buf.data())[(size_t)a * n + m];
     };
@@ -632,7 +747,7 @@ int qk_get_particles(qk_ctx *c, int s, int64_t cap, double *rec, int32_t *ids, i
         w[7] = get(6, m);
         w[8] = b.cmr;
         w[9] = get(7, m);
-        if (ids) ids[o] = b.id ? id[(size_t)m] : 0;
+        if (ids) ids[o] = src.id ? id[(size_t)m] : 0;
         ++o;
     }
     *n_out = o;
@@ -645,13 +760,13 @@ int qk_compute_rho(qk_ctx *c, double *rho)
     const size_t bytes = (size_t)c->g.ncell * 3 * sizeof(double);
     double *d = nullptr;
     HIP_TRY(hipMalloc(&d, bytes));
-    hipError_t e = hipMemsetAsync(d, 0, bytes, c->stream);
+    hipError_t e = launch_zero(c->stream, d, (size_t)c->g.ncell * 3);
     const int slots[3] = {QK_ELECTRON, QK_POSITRON, QK_PHOTON};
     for (int t = 0; t < 3 && e == hipSuccess; ++t) {
         SpeciesBuf &b = c->sp[slots[t]];
         if (!b.active || b.n == 0) continue;
-        e = c->rsize == 4 ? launch_compute_rho<float>(c->stream, c->g, species_of<float>(b), b.n, d, t)
-                          : launch_compute_rho<double>(c->stream, c->g, species_of<double>(b), b.n, d, t);
+        e = c->rsize == 4 ? launch_compute_rho<float>(c->stream, c->g, view<float>(b.buf[b.cur]), b.n, d, t)
+                          : launch_compute_rho<double>(c->stream, c->g, view<double>(b.buf[b.cur]), b.n, d, t);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(rho, d, bytes, hipMemcpyDeviceToHost);
@@ -667,6 +782,13 @@ int qk_last_timing(qk_ctx *c, double *ms_total, double *ms_particles, double *ms
     if (ms_total) *ms_total = c->ms_total;
     if (ms_particles) *ms_particles = c->ms_part;
     if (ms_fields) *ms_fields = c->ms_field;
+    return QK_OK;
+}
+
+int qk_last_fallback(qk_ctx *c, double *fraction)
+{
+    if (!c || !fraction) return fail(QK_ERR_INVALID, "null argument");
+    *fraction = c->fallback_frac;
     return QK_OK;
 }
 

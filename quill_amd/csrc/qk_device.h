@@ -27,6 +27,30 @@ __host__ __device__ inline int cell_i(int c) { return c >> 20; }
 __host__ __device__ inline int cell_j(int c) { return (c >> 10) & 1023; }
 __host__ __device__ inline int cell_k(int c) { return c & 1023; }
 
+// cell tiles: 8 x 8 x 8 cells, numbered i-major; particles are sorted by
+// (tile, cell inside the tile)
+constexpr int kTileShift = 3;
+constexpr int kTile = 1 << kTileShift;
+constexpr int kTileCells = kTile * kTile * kTile;
+
+struct TileGeom {
+    int ntx, nty, ntz;   // tiles per axis
+    int ntiles;
+    int nkeys;           // ntiles * kTileCells
+};
+
+__host__ __device__ inline int tile_of(const TileGeom &t, int i, int j, int k)
+{
+    return ((i >> kTileShift) * t.nty + (j >> kTileShift)) * t.ntz + (k >> kTileShift);
+}
+
+__host__ __device__ inline int sort_key(const TileGeom &t, int c)
+{
+    const int i = c >> 20, j = (c >> 10) & 1023, k = c & 1023;
+    const int m = kTile - 1;
+    return tile_of(t, i, j, k) * kTileCells + (((i & m) << (2 * kTileShift)) | ((j & m) << kTileShift) | (k & m));
+}
+
 // is_inside (src/plist.cpp:42-48)
 __device__ inline bool is_inside(const Grid &g, int i, int j, int k)
 {

@@ -15,28 +15,42 @@ struct SolverCoef {
 };
 
 // per-species accumulators of the fused diagnostics, all in FP64:
-// [0] sum q, [1] sum energy, [2] count
-constexpr int kDiagPerSpecies = 3;
+// [0] sum q, [1] sum energy, [2] live count, [3] particles deposited through
+// the global fallback of the tiled kernel
+constexpr int kDiagPerSpecies = 4;
+
+// particles per workgroup of the tiled kernel (a chunk never spans two tiles)
+constexpr int kChunk = 4096;
 
 struct PushArgs {
-    int solver_unused;
     int pusher;              // 0 Vay, 1 Boris
-    int kind;                // 0 lepton / ion (charged), 1 photon
-    int deposit;             // QK_DEPOSIT_*
+    int kind;                // 0 charged (lepton / ion), 1 photon
     double cmr;              // charge-to-mass ratio
     double external_bz;
-    int64_t n;               // slots to visit
+    int64_t first, n;        // slots [first, first + n) (global kernel)
     double *diag;            // kDiagPerSpecies doubles, accumulated atomically
     int energy_mode;         // 0: (g-1) q ; 1: g q (photons) ; 2: (g-1) q / cmr (ions)
+    // tiled kernel: work list of (tile, first slot, count) chunks
+    const int4 *chunks;
+    const int *nchunks;      // device counter
+    int max_chunks;          // launch size (upper bound of *nchunks)
+    TileGeom tg;
 };
 
-// clear n doubles with a kernel (kept out of hipMemsetAsync so the step can be
-// captured in a HIP graph)
+// clear n doubles / ints with a kernel (kept out of hipMemsetAsync so the
+// step can be captured in a HIP graph)
 hipError_t launch_zero(hipStream_t st, double *p, size_t n);
+hipError_t launch_fill_int(hipStream_t st, int *p, size_t n, int v);
 
+// one thread per particle, every current contribution a global atomic
 template <typename real>
-hipError_t launch_push(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
-                       const Species<real> &s, const PushArgs &a);
+hipError_t launch_push_global(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
+                              const Species<real> &s, const PushArgs &a);
+// one workgroup per chunk of a cell tile: the current is accumulated in an
+// LDS window around the tile and flushed to global memory once
+template <typename real>
+hipError_t launch_push_tiled(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
+                             const Species<real> &s, const PushArgs &a);
 
 template <typename real>
 hipError_t launch_maxwell_b(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f);
@@ -56,5 +70,29 @@ hipError_t launch_load_uniform(hipStream_t st, const Grid &g, const Species<real
                                double q, double sigma, uint64_t seed, int ilo, int ihi);
 template <typename real>
 hipError_t launch_compute_rho(hipStream_t st, const Grid &g, const Species<real> &s, int64_t n, double *rho, int slot);
+
+// ---- cell-tile counting sort ----------------------------------------------
+
+// scratch for the exclusive scan of n ints: at most scan_scratch_ints(n) ints
+size_t scan_scratch_ints(size_t n);
+// out[i] = sum of in[0..i) for i in [0, n]; out has n + 1 entries
+hipError_t launch_exclusive_scan(hipStream_t st, const int *in, int *out, int n, int *scratch);
+
+struct SortBuffers {
+    int *counts;      // nkeys + 1
+    int *offsets;     // nkeys + 1 (consumed as cursors by the scatter)
+    int *tile_start;  // ntiles + 1
+    int *chunk_n;     // ntiles + 1
+    int *chunk_off;   // ntiles + 1
+    int4 *chunks;     // max_chunks
+    int *scratch;     // scan scratch
+    int max_chunks;
+};
+
+// sort the live particles of src[0, n) into dst by (tile, cell); writes
+// tile_start, the chunk work list and its length (chunk_off[ntiles])
+template <typename real>
+hipError_t launch_sort(hipStream_t st, const TileGeom &tg, const Species<real> &src, const Species<real> &dst,
+                       int64_t n, const SortBuffers &b);
 
 }  // namespace qk

@@ -1,7 +1,16 @@
 // qk_particles.hip -- particle kernels of the step on gfx950: the fused
-// gather + push + charge-conserving deposit + re-binning kernel (the particle
-// loop of src/advance.cpp:90-387), photon transport, the device plasma loader
-// and the node-charge deposit used for the Gauss-law check.
+// gather + push + charge-conserving deposit + re-binning kernels (the particle
+// loop of src/advance.cpp:90-387) in two forms, photon transport, the device
+// plasma loader and the node-charge deposit used for the Gauss-law check.
+//
+//   k_push_global  one thread per particle, each current contribution a global
+//                  atomic (the plain method, used for the FP64 comparison)
+//   k_push_tiled   one workgroup per chunk of an 8x8x8 cell tile; particles are
+//                  sorted by (tile, cell), so the wavefront's gathers hit a few
+//                  cache lines and its deposits land in an LDS window of the
+//                  tile plus a halo, flushed once with one global atomic per
+//                  nonzero window entry.  A particle that drifted out of the
+//                  window since the last sort deposits through global atomics.
 #include "qk_kernels.h"
 
 namespace qk {
@@ -9,8 +18,20 @@ namespace qk {
 namespace {
 
 constexpr int kBlock = 256;
+// the window covers start cells within kHalo cells of the tile; a deposit
+// reaches one cell below and two above the start cell
+constexpr int kHalo = 1;
+constexpr int kWin = kTile + 2 * kHalo + 3;
+constexpr int kWin3 = kWin * kWin * kWin;
 
-// every contribution is a global atomic (the plain method)
+template <typename real>
+struct StepConst {
+    real cmr_dt, ext_bz, inv_d[3], dt;
+    double inv_cmr;
+    int energy_mode;
+};
+
+// every contribution is a global atomic
 template <typename real>
 struct GlobalAcc {
     real *jx, *jy, *jz;
@@ -26,121 +47,191 @@ struct GlobalAcc {
     }
 };
 
-__device__ inline void block_accumulate(double *dst, double v0, double v1, double v2)
+// contributions go to the workgroup's LDS window [3][kWin][kWin][kWin]
+template <typename real>
+struct LdsAcc {
+    real *win;
+    int base;   // window index of the particle's start cell
+    int cur;
+    __device__ inline void set_cell(int wi, int wj, int wk) { cur = base + (wi * kWin + wj) * kWin + wk; }
+    __device__ inline void operator()(int comp, int di, int dj, int dk, real v)
+    {
+        atomicAdd(win + comp * kWin3 + cur + (di * kWin + dj) * kWin + dk, v);
+    }
+};
+
+__device__ inline void block_accumulate(double *dst, double v0, double v1, double v2, double v3)
 {
-    __shared__ double part[3][kBlock / 64];
+    __shared__ double part[4][kBlock / 64];
     v0 = wave_sum(v0);
     v1 = wave_sum(v1);
     v2 = wave_sum(v2);
+    v3 = wave_sum(v3);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         part[0][w] = v0;
         part[1][w] = v1;
         part[2][w] = v2;
+        part[3][w] = v3;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s0 = 0, s1 = 0, s2 = 0;
-        for (int m = 0; m < kBlock / 64; ++m) {
-            s0 += part[0][m];
-            s1 += part[1][m];
-            s2 += part[2][m];
-        }
-        if (s2 != 0) {
-            atomicAdd(dst + 0, s0);
-            atomicAdd(dst + 1, s1);
-            atomicAdd(dst + 2, s2);
-        }
+        double s[4] = {0, 0, 0, 0};
+        for (int m = 0; m < kBlock / 64; ++m)
+            for (int a = 0; a < 4; ++a) s[a] += part[a][m];
+        if (s[2] != 0 || s[3] != 0)
+            for (int a = 0; a < 4; ++a) atomicAdd(dst + a, s[a]);
     }
 }
 
-// Charged particles (electrons, positrons, ions) with QED off: gather E and
-// Be at the particle, add external Bz, push (Vay or Boris), deposit the
-// current of the straight path u dt / gamma split at cell faces, move, and
-// drop the particle if its new cell is not is_inside (place(),
-// src/plist.cpp:7-40).  Diagnostics of compute_N / compute_energy
-// (src/fout.cpp:41-138) are accumulated from the post-push state.
-template <typename real, int PUSHER>
-__global__ void __launch_bounds__(kBlock) k_push_global(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
-                                                        int64_t n, real cmr_dt, real ext_bz, real inv_dx, real inv_dy,
-                                                        real inv_dz, real dt, double *diag, int energy_mode,
-                                                        double inv_cmr)
+struct DiagAcc {
+    double q = 0, e = 0, n = 0, fallback = 0;
+};
+
+// One charged particle with QED off: gather E and Be at the particle, add
+// external Bz, push (Vay or Boris), deposit the current of the straight path
+// u dt / gamma split at cell faces, move, and drop the particle if its new
+// cell is not is_inside (place(), src/plist.cpp:7-40).  Diagnostics of
+// compute_N / compute_energy (src/fout.cpp:41-138) come from the post-push
+// state.  dep(base_cell_i, j, k, q, pos, d) performs the deposit and
+// advances pos.
+template <typename real, int PUSHER, class Dep>
+__device__ inline void particle_step(const Grid &g, const Fields<real> &f, const Species<real> &s, int64_t p,
+                                     const StepConst<real> &sc, Dep &&dep, DiagAcc &da)
 {
-    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    double dq = 0, de = 0, dn = 0;
-    if (p < n) {
-        const int c = s.cell[p];
-        if (c >= 0) {
-            const int i = cell_i(c), j = cell_j(c), k = cell_k(c);
-            real pos[3] = {s.ox[p], s.oy[p], s.oz[p]};
-            real u[3] = {s.ux[p], s.uy[p], s.uz[p]};
-            const real q = s.q[p];
-            const int base = i * g.sx + j * g.sy + k;
-            real e[3], b[3];
-            gather3(f.ex, f.ey, f.ez, base, g.sx, g.sy, pos[0], pos[1], pos[2], e);
-            gather3(f.bex, f.bey, f.bez, base, g.sx, g.sy, pos[0], pos[1], pos[2], b);
-            b[2] += ext_bz;
-            real gam = sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-            push<PUSHER>(cmr_dt, e, b, u, gam);
-            // displacement in cells (containers.cpp:58-67, advance.cpp:281-284)
-            const real a = dt / gam;
-            const real d[3] = {u[0] * a * inv_dx, u[1] * a * inv_dy, u[2] * a * inv_dz};
-            // |d| < dt/dx < 1 for any finite momentum; a non-finite one (corrupt
-            // input) would index outside the stencil, so the particle is dropped
-            const bool finite = fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1);
-            if (!finite) s.cell[p] = -1;
-            if (finite) {
-            GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
-            deposit_path<real>(acc, q, pos, d);
-            // re-bin: new cell and offset
-            int ci[3];
+    const int c = s.cell[p];
+    if (c < 0) return;
+    const int i = cell_i(c), j = cell_j(c), k = cell_k(c);
+    real pos[3] = {s.ox[p], s.oy[p], s.oz[p]};
+    real u[3] = {s.ux[p], s.uy[p], s.uz[p]};
+    const real q = s.q[p];
+    const int base = i * g.sx + j * g.sy + k;
+    real e[3], b[3];
+    gather3(f.ex, f.ey, f.ez, base, g.sx, g.sy, pos[0], pos[1], pos[2], e);
+    gather3(f.bex, f.bey, f.bez, base, g.sx, g.sy, pos[0], pos[1], pos[2], b);
+    b[2] += sc.ext_bz;
+    real gam = sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    push<PUSHER>(sc.cmr_dt, e, b, u, gam);
+    // displacement in cells (containers.cpp:58-67, advance.cpp:281-284)
+    const real a = sc.dt / gam;
+    const real d[3] = {u[0] * a * sc.inv_d[0], u[1] * a * sc.inv_d[1], u[2] * a * sc.inv_d[2]};
+    s.ux[p] = u[0];
+    s.uy[p] = u[1];
+    s.uz[p] = u[2];
+    // |d| < dt/dx < 1 for any finite momentum; a non-finite one (corrupt
+    // input) would index outside the stencil, so that particle is dropped
+    if (!(fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1))) {
+        s.cell[p] = -1;
+        return;
+    }
+    dep(i, j, k, base, q, pos, d, da);
+    int ci[3];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const real fl = floor(pos[m]);
-                ci[m] = (int)fl;
-                pos[m] -= fl;
-                if (pos[m] >= real(1)) {   // rounding of a tiny negative position
-                    pos[m] = real(0);
-                    ci[m] += 1;
-                }
-            }
-            const int ni = i + ci[0], nj = j + ci[1], nk = k + ci[2];
-            s.ux[p] = u[0];
-            s.uy[p] = u[1];
-            s.uz[p] = u[2];
-            if (is_inside(g, ni, nj, nk)) {
-                s.cell[p] = pack_cell(ni, nj, nk);
-                s.ox[p] = pos[0];
-                s.oy[p] = pos[1];
-                s.oz[p] = pos[2];
-                dq = (double)q;
-                dn = 1;
-                const double gg = (double)gam;
-                de = energy_mode == 2 ? (gg - 1) * (double)q * inv_cmr : (gg - 1) * (double)q;
-            } else {
-                s.cell[p] = -1;
-            }
-            }   // finite
+    for (int m = 0; m < 3; ++m) {
+        const real fl = floor(pos[m]);
+        ci[m] = (int)fl;
+        pos[m] -= fl;
+        if (pos[m] >= real(1)) {   // rounding of a tiny negative position
+            pos[m] = real(0);
+            ci[m] += 1;
         }
     }
-    block_accumulate(diag, dq, de, dn);
+    const int ni = i + ci[0], nj = j + ci[1], nk = k + ci[2];
+    if (is_inside(g, ni, nj, nk)) {
+        s.cell[p] = pack_cell(ni, nj, nk);
+        s.ox[p] = pos[0];
+        s.oy[p] = pos[1];
+        s.oz[p] = pos[2];
+        const double gg = (double)gam;
+        da.q += (double)q;
+        da.n += 1;
+        da.e += sc.energy_mode == 2 ? (gg - 1) * (double)q * sc.inv_cmr : (gg - 1) * (double)q;
+    } else {
+        s.cell[p] = -1;
+    }
+}
+
+template <typename real, int PUSHER>
+__global__ void __launch_bounds__(kBlock) k_push_global(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
+                                                        int64_t first, int64_t n, StepConst<real> sc, double *diag)
+{
+    const int64_t p = first + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    DiagAcc da;
+    if (p < first + n) {
+        auto dep = [&](int, int, int, int base, real q, real pos[3], const real d[3], DiagAcc &) {
+            GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
+            deposit_path<real>(acc, q, pos, d);
+        };
+        particle_step<real, PUSHER>(g, f, s, p, sc, dep, da);
+    }
+    block_accumulate(diag, da.q, da.e, da.n, da.fallback);
+}
+
+template <typename real, int PUSHER>
+__global__ void __launch_bounds__(kBlock) k_push_tiled(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
+                                                       StepConst<real> sc, const int4 *__restrict__ chunks,
+                                                       const int *__restrict__ nchunks, TileGeom tg, double *diag)
+{
+    __shared__ real win[3 * kWin3];
+    if ((int)blockIdx.x >= *nchunks) return;   // whole workgroup leaves together
+    const int4 ch = chunks[blockIdx.x];
+    const int tile = ch.x;
+    const int tk = tile % tg.ntz, t2 = tile / tg.ntz;
+    const int tj = t2 % tg.nty, ti = t2 / tg.nty;
+    const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
+    for (int w = threadIdx.x; w < 3 * kWin3; w += kBlock) win[w] = real(0);
+    __syncthreads();
+
+    DiagAcc da;
+    auto dep = [&](int i, int j, int k, int base, real q, real pos[3], const real d[3], DiagAcc &acc_d) {
+        const int ri = i - i0 + kHalo, rj = j - j0 + kHalo, rk = k - k0 + kHalo;
+        const int lim = kTile + 2 * kHalo;
+        if ((unsigned)ri < (unsigned)lim && (unsigned)rj < (unsigned)lim && (unsigned)rk < (unsigned)lim) {
+            // window index of the start cell: one extra layer below for w = -1
+            LdsAcc<real> acc{win, ((ri + 1) * kWin + (rj + 1)) * kWin + (rk + 1), 0};
+            deposit_path<real>(acc, q, pos, d);
+        } else {
+            GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
+            deposit_path<real>(acc, q, pos, d);
+            acc_d.fallback += 1;
+        }
+    };
+    const int64_t end = (int64_t)ch.y + ch.z;
+    for (int64_t p = (int64_t)ch.y + threadIdx.x; p < end; p += kBlock) particle_step<real, PUSHER>(g, f, s, p, sc, dep, da);
+    __syncthreads();
+
+    // flush the window: one global atomic per nonzero entry inside the grid
+    const int wi0 = i0 - kHalo - 1, wj0 = j0 - kHalo - 1, wk0 = k0 - kHalo - 1;
+    for (int w = threadIdx.x; w < kWin3; w += kBlock) {
+        const int wk = w % kWin, t = w / kWin;
+        const int wj = t % kWin, wi = t / kWin;
+        const int gi = wi0 + wi, gj = wj0 + wj, gk = wk0 + wk;
+        if ((unsigned)gi >= (unsigned)g.nx || (unsigned)gj >= (unsigned)g.ny || (unsigned)gk >= (unsigned)g.nz) continue;
+        const int id = gi * g.sx + gj * g.sy + gk;
+        const real vx = win[w], vy = win[kWin3 + w], vz = win[2 * kWin3 + w];
+        if (vx != real(0)) unsafeAtomicAdd(cur.jx + id, vx);
+        if (vy != real(0)) unsafeAtomicAdd(cur.jy + id, vy);
+        if (vz != real(0)) unsafeAtomicAdd(cur.jz + id, vz);
+    }
+    block_accumulate(diag, da.q, da.e, da.n, da.fallback);
 }
 
 // Photons with QED off move on straight lines (src/advance.cpp:173-183) and
 // carry no current.
 template <typename real>
-__global__ void __launch_bounds__(kBlock) k_photon_move(Grid g, Species<real> s, int64_t n, real inv_dx, real inv_dy,
-                                                        real inv_dz, real dt, double *diag)
+__global__ void __launch_bounds__(kBlock) k_photon_move(Grid g, Species<real> s, int64_t first, int64_t n,
+                                                        StepConst<real> sc, double *diag)
 {
-    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    double dq = 0, de = 0, dn = 0;
-    if (p < n) {
+    const int64_t p = first + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    DiagAcc da;
+    if (p < first + n) {
         const int c = s.cell[p];
         if (c >= 0) {
             const real u[3] = {s.ux[p], s.uy[p], s.uz[p]};
             const real gam = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-            const real a = dt / gam;
-            real pos[3] = {s.ox[p] + u[0] * a * inv_dx, s.oy[p] + u[1] * a * inv_dy, s.oz[p] + u[2] * a * inv_dz};
+            const real a = sc.dt / gam;
+            real pos[3] = {s.ox[p] + u[0] * a * sc.inv_d[0], s.oy[p] + u[1] * a * sc.inv_d[1],
+                           s.oz[p] + u[2] * a * sc.inv_d[2]};
             int ci[3];
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
@@ -160,15 +251,15 @@ __global__ void __launch_bounds__(kBlock) k_photon_move(Grid g, Species<real> s,
                 s.oy[p] = pos[1];
                 s.oz[p] = pos[2];
                 const double q = s.q[p];
-                dq = q;
-                dn = 1;
-                de = (double)gam * q;
+                da.q = q;
+                da.n = 1;
+                da.e = (double)gam * q;
             } else {
                 s.cell[p] = -1;
             }
         }
     }
-    block_accumulate(diag, dq, de, dn);
+    block_accumulate(diag, da.q, da.e, da.n, 0.0);
 }
 
 // Uniform thermal plasma: ppc particles per is_inside cell with ilo <= i < ihi,
@@ -236,27 +327,48 @@ __global__ void __launch_bounds__(kBlock) k_compute_rho(Grid g, Species<real> s,
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+template <typename real>
+StepConst<real> step_const(const Grid &g, const PushArgs &a)
+{
+    StepConst<real> sc;
+    sc.cmr_dt = (real)(a.cmr * g.dt);
+    sc.ext_bz = (real)a.external_bz;
+    sc.inv_d[0] = (real)(1.0 / g.dx);
+    sc.inv_d[1] = (real)(1.0 / g.dy);
+    sc.inv_d[2] = (real)(1.0 / g.dz);
+    sc.dt = (real)g.dt;
+    sc.inv_cmr = a.cmr != 0 ? 1.0 / a.cmr : 0.0;
+    sc.energy_mode = a.energy_mode;
+    return sc;
+}
+
 }  // namespace
 
 template <typename real>
-hipError_t launch_push(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
-                       const Species<real> &s, const PushArgs &a)
+hipError_t launch_push_global(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
+                              const Species<real> &s, const PushArgs &a)
 {
     if (a.n <= 0) return hipSuccess;
-    const real dt = (real)g.dt;
-    const real idx = (real)(1.0 / g.dx), idy = (real)(1.0 / g.dy), idz = (real)(1.0 / g.dz);
-    if (a.kind == 1) {
-        k_photon_move<real><<<blocks_for(a.n), kBlock, 0, st>>>(g, s, a.n, idx, idy, idz, dt, a.diag);
-        return hipGetLastError();
-    }
-    const real cmr_dt = (real)(a.cmr * g.dt);
-    const double inv_cmr = a.cmr != 0 ? 1.0 / a.cmr : 0.0;
-    if (a.pusher == 1)
-        k_push_global<real, 1><<<blocks_for(a.n), kBlock, 0, st>>>(g, f, j, s, a.n, cmr_dt, (real)a.external_bz, idx,
-                                                                   idy, idz, dt, a.diag, a.energy_mode, inv_cmr);
+    const StepConst<real> sc = step_const<real>(g, a);
+    if (a.kind == 1)
+        k_photon_move<real><<<blocks_for(a.n), kBlock, 0, st>>>(g, s, a.first, a.n, sc, a.diag);
+    else if (a.pusher == 1)
+        k_push_global<real, 1><<<blocks_for(a.n), kBlock, 0, st>>>(g, f, j, s, a.first, a.n, sc, a.diag);
     else
-        k_push_global<real, 0><<<blocks_for(a.n), kBlock, 0, st>>>(g, f, j, s, a.n, cmr_dt, (real)a.external_bz, idx,
-                                                                   idy, idz, dt, a.diag, a.energy_mode, inv_cmr);
+        k_push_global<real, 0><<<blocks_for(a.n), kBlock, 0, st>>>(g, f, j, s, a.first, a.n, sc, a.diag);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_push_tiled(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
+                             const Species<real> &s, const PushArgs &a)
+{
+    if (a.max_chunks <= 0) return hipSuccess;
+    const StepConst<real> sc = step_const<real>(g, a);
+    if (a.pusher == 1)
+        k_push_tiled<real, 1><<<a.max_chunks, kBlock, 0, st>>>(g, f, j, s, sc, a.chunks, a.nchunks, a.tg, a.diag);
+    else
+        k_push_tiled<real, 0><<<a.max_chunks, kBlock, 0, st>>>(g, f, j, s, sc, a.chunks, a.nchunks, a.tg, a.diag);
     return hipGetLastError();
 }
 
@@ -279,8 +391,10 @@ hipError_t launch_compute_rho(hipStream_t st, const Grid &g, const Species<real>
 }
 
 #define QK_INST(real)                                                                                            \
-    template hipError_t launch_push<real>(hipStream_t, const Grid &, const Fields<real> &, const Current<real> &,  \
-                                          const Species<real> &, const PushArgs &);                              \
+    template hipError_t launch_push_global<real>(hipStream_t, const Grid &, const Fields<real> &,               \
+                                                 const Current<real> &, const Species<real> &, const PushArgs &); \
+    template hipError_t launch_push_tiled<real>(hipStream_t, const Grid &, const Fields<real> &,                \
+                                                const Current<real> &, const Species<real> &, const PushArgs &); \
     template hipError_t launch_load_uniform<real>(hipStream_t, const Grid &, const Species<real> &, int64_t, int, \
                                                   double, double, uint64_t, int, int);                           \
     template hipError_t launch_compute_rho<real>(hipStream_t, const Grid &, const Species<real> &, int64_t,       \

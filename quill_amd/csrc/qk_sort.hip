@@ -1,0 +1,195 @@
+// qk_sort.hip -- cell-tile counting sort of one particle species on gfx950
+// (replaces the per-cell linked lists and place() re-linking of
+// src/plist.cpp:7-40 and the page allocator of src/low-level.cpp:186-286).
+//
+//   1. count live particles per key (tile, cell in tile)
+//   2. exclusive scan of the counts -> first slot of every key
+//   3. scatter every live particle to its key's next slot in the other buffer
+//      (free slots are dropped: this is also the compaction)
+//   4. per tile: first slot and the number of kChunk-particle chunks, scanned
+//      into the work list of the tiled push kernel
+#include "qk_kernels.h"
+
+namespace qk {
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__global__ void k_fill_int(int *p, size_t n, int v)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// exclusive scan of kScanTile items per workgroup; block totals to sums
+__global__ void __launch_bounds__(kScanBlock) k_scan_tile(const int *__restrict__ in, int *__restrict__ out, int n,
+                                                          int *__restrict__ sums)
+{
+    __shared__ int wsum[kScanBlock / 64];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int local = 0;
+#pragma unroll
+    for (int m = 0; m < kScanItems; ++m) {
+        v[m] = (base + m < n) ? in[base + m] : 0;
+        local += v[m];
+    }
+    // inclusive scan of the thread totals inside the wavefront
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int wprefix = 0;
+    for (int w = 0; w < wave; ++w) wprefix += wsum[w];
+    int run = wprefix + x - local;   // exclusive prefix of this thread
+#pragma unroll
+    for (int m = 0; m < kScanItems; ++m) {
+        if (base + m < n) out[base + m] = run;
+        run += v[m];
+    }
+    if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = wprefix + x;
+}
+
+__global__ void k_scan_add(int *out, int n, const int *__restrict__ offs)
+{
+    const int t = blockIdx.x * kScanTile + threadIdx.x;
+    const int add = offs[blockIdx.x];
+    for (int m = 0; m < kScanItems; ++m) {
+        const int id = t + m * kScanBlock;
+        if (id < n) out[id] += add;
+    }
+}
+
+__global__ void k_count_keys(const int *__restrict__ cell, int64_t n, TileGeom tg, int *counts)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int c = cell[p];
+    if (c >= 0) atomicAdd(counts + sort_key(tg, c), 1);
+}
+
+template <typename real>
+__global__ void k_scatter(Species<real> src, Species<real> dst, int64_t n, TileGeom tg, int *cursor)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int c = src.cell[p];
+    if (c < 0) return;
+    const int o = atomicAdd(cursor + sort_key(tg, c), 1);
+    dst.cell[o] = c;
+    dst.ox[o] = src.ox[p];
+    dst.oy[o] = src.oy[p];
+    dst.oz[o] = src.oz[p];
+    dst.ux[o] = src.ux[p];
+    dst.uy[o] = src.uy[p];
+    dst.uz[o] = src.uz[p];
+    dst.q[o] = src.q[p];
+    if (src.chi) dst.chi[o] = src.chi[p];
+    if (src.id) dst.id[o] = src.id[p];
+}
+
+__global__ void k_tile_chunks(const int *__restrict__ offsets, int ntiles, int *tile_start, int *chunk_n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    const int s = offsets[t * kTileCells];
+    tile_start[t] = s;
+    if (t < ntiles) {
+        const int cnt = offsets[(t + 1) * kTileCells] - s;
+        chunk_n[t] = (cnt + kChunk - 1) / kChunk;
+    } else {
+        chunk_n[t] = 0;
+    }
+}
+
+__global__ void k_write_chunks(const int *__restrict__ tile_start, const int *__restrict__ chunk_off, int ntiles,
+                               int4 *chunks, int max_chunks)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int s = tile_start[t], e = tile_start[t + 1];
+    int o = chunk_off[t];
+    for (int p = s; p < e && o < max_chunks; p += kChunk, ++o) chunks[o] = make_int4(t, p, min(kChunk, e - p), 0);
+}
+
+inline unsigned nblocks(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_fill_int(hipStream_t st, int *p, size_t n, int v)
+{
+    if (n == 0) return hipSuccess;
+    k_fill_int<<<nblocks(n, 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+size_t scan_scratch_ints(size_t n)
+{
+    // mirrors the recursion of launch_exclusive_scan: a level over m items
+    // uses 2 (nb + 1) ints and recurses on nb items
+    size_t total = 0, m = n + 1;
+    for (;;) {
+        const size_t nb = (m + kScanTile - 1) / kScanTile;
+        total += 2 * (nb + 1);
+        if (nb == 1) break;
+        m = nb + 1;
+    }
+    return total + 16;
+}
+
+// out[0..n] = exclusive prefix sums of in[0..n), out[n] = total
+hipError_t launch_exclusive_scan(hipStream_t st, const int *in, int *out, int n, int *scratch)
+{
+    // the total lands in out[n] by scanning n + 1 items with a trailing zero;
+    // in[n] must be readable and zero (callers size their arrays n + 1)
+    const int m = n + 1;
+    const int nb = (m + kScanTile - 1) / kScanTile;
+    int *sums = scratch, *sums_scan = scratch + nb + 1;
+    k_scan_tile<<<nb, kScanBlock, 0, st>>>(in, out, m, sums);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nb == 1) return e;
+    // the block totals are scanned recursively; their array gets a zero tail
+    k_fill_int<<<1, 1, 0, st>>>(sums + nb, 1, 0);
+    e = launch_exclusive_scan(st, sums, sums_scan, nb, scratch + 2 * (nb + 1));
+    if (e != hipSuccess) return e;
+    k_scan_add<<<nb, kScanBlock, 0, st>>>(out, m, sums_scan);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_sort(hipStream_t st, const TileGeom &tg, const Species<real> &src, const Species<real> &dst,
+                       int64_t n, const SortBuffers &b)
+{
+    hipError_t e = launch_fill_int(st, b.counts, (size_t)tg.nkeys + 1, 0);
+    if (e != hipSuccess) return e;
+    if (n > 0) {
+        k_count_keys<<<nblocks((size_t)n, 256), 256, 0, st>>>(src.cell, n, tg, b.counts);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if ((e = launch_exclusive_scan(st, b.counts, b.offsets, tg.nkeys, b.scratch)) != hipSuccess) return e;
+    k_tile_chunks<<<nblocks((size_t)tg.ntiles + 1, 256), 256, 0, st>>>(b.offsets, tg.ntiles, b.tile_start, b.chunk_n);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (n > 0) {
+        k_scatter<real><<<nblocks((size_t)n, 256), 256, 0, st>>>(src, dst, n, tg, b.offsets);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if ((e = launch_exclusive_scan(st, b.chunk_n, b.chunk_off, tg.ntiles, b.scratch)) != hipSuccess) return e;
+    k_write_chunks<<<nblocks((size_t)tg.ntiles, 256), 256, 0, st>>>(b.tile_start, b.chunk_off, tg.ntiles, b.chunks,
+                                                                    b.max_chunks);
+    return hipGetLastError();
+}
+
+template hipError_t launch_sort<float>(hipStream_t, const TileGeom &, const Species<float> &, const Species<float> &,
+                                       int64_t, const SortBuffers &);
+template hipError_t launch_sort<double>(hipStream_t, const TileGeom &, const Species<double> &,
+                                        const Species<double> &, int64_t, const SortBuffers &);
+
+}  // namespace qk
