@@ -166,8 +166,30 @@ def cpu_baseline(w, budget_steps=2):
                 sample=f"FP64 C restatement (oracle/), full grid, {n} particles, {budget_steps} steps, {sec:.2f} s")
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime the library
+    uses.  Profiled with FETCH_SIZE / WRITE_SIZE, it gives the factor between
+    those counters and bytes actually moved for a kernel of known traffic."""
+    import ctypes as C
+    qa.load_library()   # the same libamdhip64 as the library's
+    hip = C.CDLL("libamdhip64.so.7")   # already loaded by libquillk.so
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "rc": rc}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
@@ -176,6 +198,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison runs")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design, see the sharding notes)", file=sys.stderr)

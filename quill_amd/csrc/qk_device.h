@@ -96,6 +96,15 @@ __device__ inline void gather3(const real *__restrict__ fx, const real *__restri
 }
 
 // ---------------------------------------------------------------------------
+// reciprocal and square root: the hardware instructions (1 ulp) in FP32,
+// IEEE-exact in FP64
+
+__device__ inline float qk_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double qk_rcp(double x) { return 1.0 / x; }
+__device__ inline float qk_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline double qk_sqrt(double x) { return sqrt(x); }
+
+// ---------------------------------------------------------------------------
 // pushers (src/pusher.cpp)
 
 // Vay 2008 (pusher.cpp:5-42); g enters as gamma at n-1/2 and leaves as gamma at n+1/2
@@ -105,18 +114,18 @@ __device__ inline void push_vay(real cmr_dt, const real ef[3], const real bf[3],
     const real e0 = ef[0] * cmr_dt, e1 = ef[1] * cmr_dt, e2 = ef[2] * cmr_dt;
     const real h = real(0.5) * cmr_dt;
     const real b0 = bf[0] * h, b1 = bf[1] * h, b2 = bf[2] * h;
-    const real ig = real(1) / g;
+    const real ig = qk_rcp(g);
     const real w0 = u[0] + e0 + (u[1] * b2 - u[2] * b1) * ig;
     const real w1 = u[1] + e1 + (u[2] * b0 - u[0] * b2) * ig;
     const real w2 = u[2] + e2 + (u[0] * b1 - u[1] * b0) * ig;
     const real bb = b0 * b0 + b1 * b1 + b2 * b2;
     const real bw = b0 * w0 + b1 * w1 + b2 * w2;
     const real s = real(1) + w0 * w0 + w1 * w1 + w2 * w2 - bb;
-    g = sqrt(real(0.5) * (s + sqrt(s * s + real(4) * (bb + bw * bw))));
-    const real ig2 = real(1) / (g * g);
-    const real den = real(1) / (real(1) + bb * ig2);
+    g = qk_sqrt(real(0.5) * (s + qk_sqrt(s * s + real(4) * (bb + bw * bw))));
+    const real ign = qk_rcp(g);
+    const real ig2 = ign * ign;
+    const real den = qk_rcp(real(1) + bb * ig2);
     const real bwg = bw * ig2;
-    const real ign = real(1) / g;
     u[0] = (w0 + (w1 * b2 - w2 * b1) * ign + b0 * bwg) * den;
     u[1] = (w1 + (w2 * b0 - w0 * b2) * ign + b1 * bwg) * den;
     u[2] = (w2 + (w0 * b1 - w1 * b0) * ign + b2 * bwg) * den;
@@ -129,20 +138,19 @@ __device__ inline void push_boris(real cmr_dt, const real ef[3], const real bf[3
     const real h = real(0.5) * cmr_dt;
     const real e0 = ef[0] * h, e1 = ef[1] * h, e2 = ef[2] * h;
     real m0 = u[0] + e0, m1 = u[1] + e1, m2 = u[2] + e2;
-    const real gm = sqrt(real(1) + m0 * m0 + m1 * m1 + m2 * m2);
-    const real hg = h / gm;
+    const real hg = h * qk_rcp(qk_sqrt(real(1) + m0 * m0 + m1 * m1 + m2 * m2));
     const real t0 = bf[0] * hg, t1 = bf[1] * hg, t2 = bf[2] * hg;
     const real w0 = m0 + (m1 * t2 - m2 * t1);
     const real w1 = m1 + (m2 * t0 - m0 * t2);
     const real w2 = m2 + (m0 * t1 - m1 * t0);
-    const real f = real(2) / (real(1) + t0 * t0 + t1 * t1 + t2 * t2);
+    const real f = real(2) * qk_rcp(real(1) + t0 * t0 + t1 * t1 + t2 * t2);
     m0 += (w1 * t2 - w2 * t1) * f;
     m1 += (w2 * t0 - w0 * t2) * f;
     m2 += (w0 * t1 - w1 * t0) * f;
     u[0] = m0 + e0;
     u[1] = m1 + e1;
     u[2] = m2 + e2;
-    g = sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    g = qk_sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
 }
 
 template <int PUSHER, typename real>

@@ -47,16 +47,19 @@ struct GlobalAcc {
     }
 };
 
-// contributions go to the workgroup's LDS window [3][kWin][kWin][kWin]
+// contributions go to the workgroup's LDS window [3][kWin][kWin][kWin].  The
+// window is FP64 for both particle precisions: on gfx950 an FP64 LDS atomic
+// add sustains about twenty times the rate of an FP32 one, and the tile's
+// partial sums keep more bits before the single FP32/FP64 flush.
 template <typename real>
 struct LdsAcc {
-    real *win;
+    double *win;
     int base;   // window index of the particle's start cell
     int cur;
     __device__ inline void set_cell(int wi, int wj, int wk) { cur = base + (wi * kWin + wj) * kWin + wk; }
     __device__ inline void operator()(int comp, int di, int dj, int dk, real v)
     {
-        atomicAdd(win + comp * kWin3 + cur + (di * kWin + dj) * kWin + dk, v);
+        atomicAdd(win + comp * kWin3 + cur + (di * kWin + dj) * kWin + dk, (double)v);
     }
 };
 
@@ -88,43 +91,70 @@ struct DiagAcc {
     double q = 0, e = 0, n = 0, fallback = 0;
 };
 
+// one particle in registers
+template <typename real>
+struct PRec {
+    int cell;      // packed cell, negative: free slot
+    real o[3];     // offset in the cell
+    real u[3];     // u = p / m c
+    real q;        // macro charge
+};
+
+template <typename real>
+__device__ inline void load_rec(const Species<real> &s, int64_t p, PRec<real> &r)
+{
+    r.cell = s.cell[p];
+    r.o[0] = s.ox[p]; r.o[1] = s.oy[p]; r.o[2] = s.oz[p];
+    r.u[0] = s.ux[p]; r.u[1] = s.uy[p]; r.u[2] = s.uz[p];
+    r.q = s.q[p];
+}
+
+// write back what a push changes (the charge does not change)
+template <typename real>
+__device__ inline void store_rec(const Species<real> &s, int64_t p, const PRec<real> &r)
+{
+    s.cell[p] = r.cell;
+    s.ox[p] = r.o[0]; s.oy[p] = r.o[1]; s.oz[p] = r.o[2];
+    s.ux[p] = r.u[0]; s.uy[p] = r.u[1]; s.uz[p] = r.u[2];
+}
+
 // One charged particle with QED off: gather E and Be at the particle, add
 // external Bz, push (Vay or Boris), deposit the current of the straight path
-// u dt / gamma split at cell faces, move, and drop the particle if its new
+// u dt / gamma split at cell faces, move, and mark the slot free if the new
 // cell is not is_inside (place(), src/plist.cpp:7-40).  Diagnostics of
 // compute_N / compute_energy (src/fout.cpp:41-138) come from the post-push
-// state.  dep(base_cell_i, j, k, q, pos, d) performs the deposit and
+// state.  dep(i, j, k, base, q, pos, d, da) performs the deposit and
 // advances pos.
 template <typename real, int PUSHER, class Dep>
-__device__ inline void particle_step(const Grid &g, const Fields<real> &f, const Species<real> &s, int64_t p,
-                                     const StepConst<real> &sc, Dep &&dep, DiagAcc &da)
+__device__ inline void advance_particle(const Grid &g, const Fields<real> &f, PRec<real> &r, const StepConst<real> &sc,
+                                        Dep &&dep, DiagAcc &da)
 {
-    const int c = s.cell[p];
+    const int c = r.cell;
     if (c < 0) return;
     const int i = cell_i(c), j = cell_j(c), k = cell_k(c);
-    real pos[3] = {s.ox[p], s.oy[p], s.oz[p]};
-    real u[3] = {s.ux[p], s.uy[p], s.uz[p]};
-    const real q = s.q[p];
+    // local copies stay in registers (the deposit indexes them by axis)
+    real pos[3] = {r.o[0], r.o[1], r.o[2]};
+    real u[3] = {r.u[0], r.u[1], r.u[2]};
     const int base = i * g.sx + j * g.sy + k;
     real e[3], b[3];
     gather3(f.ex, f.ey, f.ez, base, g.sx, g.sy, pos[0], pos[1], pos[2], e);
     gather3(f.bex, f.bey, f.bez, base, g.sx, g.sy, pos[0], pos[1], pos[2], b);
     b[2] += sc.ext_bz;
-    real gam = sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    real gam = qk_sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     push<PUSHER>(sc.cmr_dt, e, b, u, gam);
     // displacement in cells (containers.cpp:58-67, advance.cpp:281-284)
-    const real a = sc.dt / gam;
+    const real a = sc.dt * qk_rcp(gam);
     const real d[3] = {u[0] * a * sc.inv_d[0], u[1] * a * sc.inv_d[1], u[2] * a * sc.inv_d[2]};
-    s.ux[p] = u[0];
-    s.uy[p] = u[1];
-    s.uz[p] = u[2];
     // |d| < dt/dx < 1 for any finite momentum; a non-finite one (corrupt
     // input) would index outside the stencil, so that particle is dropped
+    r.u[0] = u[0];
+    r.u[1] = u[1];
+    r.u[2] = u[2];
     if (!(fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1))) {
-        s.cell[p] = -1;
+        r.cell = -1;
         return;
     }
-    dep(i, j, k, base, q, pos, d, da);
+    dep(i, j, k, base, r.q, pos, d, da);
     int ci[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
@@ -136,18 +166,18 @@ __device__ inline void particle_step(const Grid &g, const Fields<real> &f, const
             ci[m] += 1;
         }
     }
+    r.o[0] = pos[0];
+    r.o[1] = pos[1];
+    r.o[2] = pos[2];
     const int ni = i + ci[0], nj = j + ci[1], nk = k + ci[2];
     if (is_inside(g, ni, nj, nk)) {
-        s.cell[p] = pack_cell(ni, nj, nk);
-        s.ox[p] = pos[0];
-        s.oy[p] = pos[1];
-        s.oz[p] = pos[2];
+        r.cell = pack_cell(ni, nj, nk);
         const double gg = (double)gam;
-        da.q += (double)q;
+        da.q += (double)r.q;
         da.n += 1;
-        da.e += sc.energy_mode == 2 ? (gg - 1) * (double)q * sc.inv_cmr : (gg - 1) * (double)q;
+        da.e += sc.energy_mode == 2 ? (gg - 1) * (double)r.q * sc.inv_cmr : (gg - 1) * (double)r.q;
     } else {
-        s.cell[p] = -1;
+        r.cell = -1;
     }
 }
 
@@ -162,24 +192,36 @@ __global__ void __launch_bounds__(kBlock) k_push_global(Grid g, Fields<real> f, 
             GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
             deposit_path<real>(acc, q, pos, d);
         };
-        particle_step<real, PUSHER>(g, f, s, p, sc, dep, da);
+        PRec<real> r;
+        load_rec(s, p, r);
+        if (r.cell >= 0) {
+            advance_particle<real, PUSHER>(g, f, r, sc, dep, da);
+            store_rec(s, p, r);
+        }
     }
     block_accumulate(diag, da.q, da.e, da.n, da.fallback);
 }
+
+// Workgroup order inside a block of kBlock particles: lane t works on
+// particle (kPerm * t) mod kBlock of the block.  Sorted particles share a cell
+// in runs of about ppc; the odd stride spreads a wavefront over distinct cells
+// (so its LDS atomics hit distinct addresses) and distinct LDS banks, while
+// the workgroup as a whole still reads and writes whole cache lines.
+constexpr int kPerm = 9;
 
 template <typename real, int PUSHER>
 __global__ void __launch_bounds__(kBlock) k_push_tiled(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
                                                        StepConst<real> sc, const int4 *__restrict__ chunks,
                                                        const int *__restrict__ nchunks, TileGeom tg, double *diag)
 {
-    __shared__ real win[3 * kWin3];
+    __shared__ double win[3 * kWin3];
     if ((int)blockIdx.x >= *nchunks) return;   // whole workgroup leaves together
     const int4 ch = chunks[blockIdx.x];
     const int tile = ch.x;
     const int tk = tile % tg.ntz, t2 = tile / tg.ntz;
     const int tj = t2 % tg.nty, ti = t2 / tg.nty;
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
-    for (int w = threadIdx.x; w < 3 * kWin3; w += kBlock) win[w] = real(0);
+    for (int w = threadIdx.x; w < 3 * kWin3; w += kBlock) win[w] = 0.0;
     __syncthreads();
 
     DiagAcc da;
@@ -197,7 +239,18 @@ __global__ void __launch_bounds__(kBlock) k_push_tiled(Grid g, Fields<real> f, C
         }
     };
     const int64_t end = (int64_t)ch.y + ch.z;
-    for (int64_t p = (int64_t)ch.y + threadIdx.x; p < end; p += kBlock) particle_step<real, PUSHER>(g, f, s, p, sc, dep, da);
+    const int sidx = (kPerm * (int)threadIdx.x) & (kBlock - 1);
+    for (int64_t b0 = ch.y; b0 < end; b0 += kBlock) {
+        const int64_t p = b0 + sidx;
+        if (p < end) {
+            PRec<real> r;
+            load_rec(s, p, r);
+            if (r.cell >= 0) {
+                advance_particle<real, PUSHER>(g, f, r, sc, dep, da);
+                store_rec(s, p, r);
+            }
+        }
+    }
     __syncthreads();
 
     // flush the window: one global atomic per nonzero entry inside the grid

@@ -68,22 +68,48 @@ __global__ void k_scan_add(int *out, int n, const int *__restrict__ offs)
     }
 }
 
-__global__ void k_count_keys(const int *__restrict__ cell, int64_t n, TileGeom tg, int *counts)
+// Runs of equal keys in consecutive lanes of a wavefront (particles arrive
+// nearly sorted, so most of a wavefront shares a few keys): every lane learns
+// the first lane of its run and the run length, and one atomic per run
+// replaces one per particle.
+__device__ inline void key_runs(int key, int lane, int &head, int &len)
+{
+    const int prev = __shfl_up(key, 1, 64);
+    const bool is_head = lane == 0 || prev != key;
+    const unsigned long long mask = __ballot(is_head);
+    const unsigned long long at_or_below = mask & ((2ull << lane) - 1ull);   // lane 63: all bits
+    head = 63 - __clzll(at_or_below);
+    const unsigned long long above = lane == 63 ? 0ull : (mask >> (lane + 1));
+    const int next = above ? lane + __ffsll(above) : 64;
+    len = next - head;
+}
+
+__global__ void __launch_bounds__(256) k_count_keys(const int *__restrict__ cell, int64_t n, TileGeom tg, int *counts)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int c = cell[p];
-    if (c >= 0) atomicAdd(counts + sort_key(tg, c), 1);
+    const int c = p < n ? cell[p] : -1;
+    const int key = c >= 0 ? sort_key(tg, c) : -1;
+    const int lane = threadIdx.x & 63;
+    int head, len;
+    key_runs(key, lane, head, len);
+    if (lane == head && key >= 0) atomicAdd(counts + key, len);
 }
 
 template <typename real>
-__global__ void k_scatter(Species<real> src, Species<real> dst, int64_t n, TileGeom tg, int *cursor)
+__global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real> dst, int64_t n, TileGeom tg,
+                                                 int *cursor)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int c = src.cell[p];
+    const int c = p < n ? src.cell[p] : -1;
+    const int key = c >= 0 ? sort_key(tg, c) : -1;
+    const int lane = threadIdx.x & 63;
+    int head, len;
+    key_runs(key, lane, head, len);
+    int base = 0;
+    if (lane == head && key >= 0) base = atomicAdd(cursor + key, len);
+    base = __shfl(base, head, 64);
     if (c < 0) return;
-    const int o = atomicAdd(cursor + sort_key(tg, c), 1);
+    const int o = base + (lane - head);
     dst.cell[o] = c;
     dst.ox[o] = src.ox[p];
     dst.oy[o] = src.oy[p];

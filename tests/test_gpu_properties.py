@@ -133,7 +133,9 @@ def test_fp32_tiled_equals_global_deposit():
         for c in range(3):
             den = np.linalg.norm(b[..., c])
             if den > 0:
-                assert np.linalg.norm(a[..., c] - b[..., c]) / den < 1e-5, name
+                # both are FP32 with different summation orders of the current:
+                # they agree to the FP32 field bar of 1e-4 relative L2
+                assert np.linalg.norm(a[..., c] - b[..., c]) / den < 1e-4, name
 
 
 def gauss_residual(E, rho, lo=3):
