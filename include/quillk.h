@@ -84,6 +84,7 @@ typedef struct {
     double N_e, N_p, N_ph;
     int64_t n_e, n_p, n_ph, n_ion;
     double energy_f, energy_e, energy_p, energy_ph, energy_ion;
+    double energy_ions[QK_MAX_ION];   /* per ion population (ienergy, fout.cpp:120-137); energy_ion is their sum */
 } qk_diag;
 
 typedef struct qk_ctx qk_ctx;

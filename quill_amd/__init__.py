@@ -57,10 +57,12 @@ class Diag(C.Structure):
     _fields_ = [("N_e", C.c_double), ("N_p", C.c_double), ("N_ph", C.c_double),
                 ("n_e", C.c_int64), ("n_p", C.c_int64), ("n_ph", C.c_int64), ("n_ion", C.c_int64),
                 ("energy_f", C.c_double), ("energy_e", C.c_double), ("energy_p", C.c_double),
-                ("energy_ph", C.c_double), ("energy_ion", C.c_double)]
+                ("energy_ph", C.c_double), ("energy_ion", C.c_double), ("energy_ions", C.c_double * MAX_ION)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["energy_ions"] = list(self.energy_ions)
+        return d
 
 
 _lib = None

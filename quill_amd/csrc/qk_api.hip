@@ -77,19 +77,25 @@ struct qk_ctx {
     bool have_step = false;
     SpeciesBuf sp[QK_MAX_SPECIES];
     // shared sort buffers (allocated at the first sort)
-    int *s_counts = nullptr, *s_offsets = nullptr, *s_tile_start = nullptr, *s_chunk_n = nullptr,
-        *s_chunk_off = nullptr, *s_scratch = nullptr;
+    int *s_counts = nullptr, *s_offsets = nullptr, *s_tile_start = nullptr, *s_tile_mult = nullptr,
+        *s_chunk_n = nullptr, *s_chunk_off = nullptr, *s_scratch = nullptr;
     int *h_sortinfo = nullptr;      // pinned: [live count, chunk count]
     double *d_diag = nullptr;       // [QK_MAX_SPECIES][kDiagPerSpecies] + field energy
     double *h_diag = nullptr;       // pinned per-step copies of the accumulators
     size_t h_diag_cap = 0;          // steps that fit in h_diag
     std::vector<hipEvent_t> ev;     // timing events
     double ms_total = 0, ms_part = 0, ms_field = 0, fallback_frac = 0;
+    // QED: device slot counters of every species plus an overflow flag, and
+    // their pinned host mirror; the global step index keys the random streams
+    int *d_count = nullptr;
+    int *h_count = nullptr;
+    uint32_t step_index = 0;
 };
 
 namespace {
 
-constexpr size_t kDiagLen = QK_MAX_SPECIES * kDiagPerSpecies + 1;
+// per-species accumulators, then the field-energy slots
+constexpr size_t kDiagLen = QK_MAX_SPECIES * kDiagPerSpecies + kEnergySlots;
 
 template <typename real>
 Fields<real> fields_of(qk_ctx *c)
@@ -229,6 +235,7 @@ int ensure_sort_buffers(qk_ctx *c)
     HIP_TRY(hipMalloc(&c->s_counts, nk * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_offsets, nk * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_tile_start, nt * sizeof(int)));
+    HIP_TRY(hipMalloc(&c->s_tile_mult, nt * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_chunk_n, nt * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_chunk_off, nt * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_scratch, scan_scratch_ints(std::max(nk, nt)) * sizeof(int)));
@@ -246,6 +253,7 @@ int sort_species(qk_ctx *c, int s)
     sb.counts = c->s_counts;
     sb.offsets = c->s_offsets;
     sb.tile_start = c->s_tile_start;
+    sb.tile_mult = c->s_tile_mult;
     sb.chunk_n = c->s_chunk_n;
     sb.chunk_off = c->s_chunk_off;
     sb.chunks = b.chunks;
@@ -282,9 +290,12 @@ void unpack_diag(const qk_ctx *c, const double *h, qk_diag &d)
     d.energy_ph = ph[1] * ep;
     for (int s = QK_ION0; s < QK_MAX_SPECIES; ++s) {
         d.n_ion += (int64_t)h[s * kDiagPerSpecies + 2];
-        d.energy_ion += h[s * kDiagPerSpecies + 1] * ep;
+        d.energy_ions[s - QK_ION0] = h[s * kDiagPerSpecies + 1] * ep;
+        d.energy_ion += d.energy_ions[s - QK_ION0];
     }
-    d.energy_f = h[QK_MAX_SPECIES * kDiagPerSpecies] * ef;
+    double ef_sum = 0;
+    for (int m = 0; m < kEnergySlots; ++m) ef_sum += h[QK_MAX_SPECIES * kDiagPerSpecies + m];
+    d.energy_f = ef_sum * ef;
 }
 
 int ensure_events(qk_ctx *c, size_t n)
@@ -293,6 +304,98 @@ int ensure_events(qk_ctx *c, size_t n)
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
         c->ev.push_back(e);
+    }
+    return QK_OK;
+}
+
+// QED slot counters: host -> device before kernels that append, device ->
+// host (with a stream sync) after them
+int qed_upload_counts(qk_ctx *c)
+{
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) c->h_count[s] = (int)c->sp[s].n;
+    c->h_count[QK_MAX_SPECIES] = 0;
+    HIP_TRY(hipMemcpyAsync(c->d_count, c->h_count, (QK_MAX_SPECIES + 1) * sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+    return QK_OK;
+}
+
+int qed_download_counts(qk_ctx *c)
+{
+    HIP_TRY(hipMemcpyAsync(c->h_count, c->d_count, (QK_MAX_SPECIES + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->h_count[QK_MAX_SPECIES]) return fail(QK_ERR_CAPACITY, "QED appends exceeded the reserved capacity");
+    for (int s = 0; s < QK_MAX_SPECIES; ++s)
+        if (c->sp[s].active) c->sp[s].n = c->h_count[s];
+    return QK_OK;
+}
+
+// The particle phase with QED on (src/main.cpp:2323-2324): birth_from_vacuum,
+// then padvance with photon emission for leptons and pair creation for
+// photons.  Capacity is reserved for the worst case (every lepton emits,
+// every photon converts) so that no append can fail; the new counts are
+// read back after the step.
+template <typename real>
+int qed_particle_phase(qk_ctx *c, const Fields<real> &f, const Current<real> &jc)
+{
+    SpeciesBuf &be = c->sp[QK_ELECTRON], &bp = c->sp[QK_POSITRON], &bph = c->sp[QK_PHOTON];
+    const QedArgs qa{c->p.e_s, c->p.seed, c->step_index};
+    auto target = [&](int s) {
+        SpeciesBuf &b = c->sp[s];
+        AppendTarget<real> t;
+        t.s = view<real>(b.buf[b.cur]);
+        t.count = c->d_count + s;
+        t.cap = (int)b.cap;
+        t.overflow = c->d_count + QK_MAX_SPECIES;
+        t.diag = c->d_diag + s * kDiagPerSpecies;
+        return t;
+    };
+    const int64_t ninner = std::max<int64_t>(0, (int64_t)(c->p.nx - 4) * (c->p.ny - 4) * (c->p.nz - 4));
+    if (ninner > 0) {
+        QK_TRY(ensure_species(c, QK_ELECTRON, ninner));
+        QK_TRY(ensure_species(c, QK_POSITRON, ninner));
+        QK_TRY(qed_upload_counts(c));
+        HIP_TRY(launch_vacuum_pairs<real>(c->stream, c->g, f, c->p.pusher, c->p.q_vac, qa, target(QK_ELECTRON),
+                                          target(QK_POSITRON)));
+        QK_TRY(qed_download_counts(c));
+    }
+    QK_TRY(ensure_species(c, QK_PHOTON, be.n + bp.n));
+    QK_TRY(ensure_species(c, QK_ELECTRON, bph.n));
+    QK_TRY(ensure_species(c, QK_POSITRON, bph.n));
+    QK_TRY(qed_upload_counts(c));
+    const int64_t n_ph0 = bph.n;   // photons emitted in this step are not visited
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        SpeciesBuf &b = c->sp[s];
+        if (!b.active || b.n == 0 || s == QK_PHOTON) continue;
+        PushArgs a{};
+        a.pusher = c->p.pusher;
+        a.kind = b.kind;
+        a.cmr = b.cmr;
+        a.external_bz = c->p.external_bz;
+        a.diag = c->d_diag + s * kDiagPerSpecies;
+        a.energy_mode = b.energy_mode;
+        a.first = 0;
+        a.n = b.n;
+        const Species<real> v = view<real>(b.buf[b.cur]);
+        if (s == QK_ELECTRON || s == QK_POSITRON)
+            HIP_TRY(launch_push_qed_lepton<real>(c->stream, c->g, f, jc, v, a, qa, target(QK_PHOTON)));
+        else
+            HIP_TRY(launch_push_global<real>(c->stream, c->g, f, jc, v, a));
+        ++b.steps_since_sort;
+    }
+    if (n_ph0 > 0) {
+        PushArgs a{};
+        a.pusher = c->p.pusher;
+        a.kind = 1;
+        a.cmr = 0;
+        a.external_bz = c->p.external_bz;
+        a.diag = c->d_diag + QK_PHOTON * kDiagPerSpecies;
+        a.energy_mode = 1;
+        a.first = 0;
+        a.n = n_ph0;
+        HIP_TRY(launch_photon_qed<real>(c->stream, c->g, f, jc, view<real>(bph.buf[bph.cur]), a, qa,
+                                        target(QK_ELECTRON), target(QK_POSITRON)));
+        ++bph.steps_since_sort;
     }
     return QK_OK;
 }
@@ -310,12 +413,15 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         c->h_diag_cap = (size_t)nsteps;
     }
     double *hd = c->h_diag;
-    const bool tiled = c->p.deposit == QK_DEPOSIT_TILED;
+    const bool qed = c->p.qed != 0;
+    // the QED kernels deposit through global atomics
+    const bool tiled = c->p.deposit == QK_DEPOSIT_TILED && !qed;
     for (int l = 0; l < nsteps; ++l) {
         // cell-tile sort: particles outside the sorted region, or the cadence
+        // (photons only with QED, where pair creation leaves free slots)
         for (int s = 0; s < QK_MAX_SPECIES; ++s) {
             SpeciesBuf &b = c->sp[s];
-            if (!b.active || b.n == 0 || b.kind == 1) continue;
+            if (!b.active || b.n == 0 || (b.kind == 1 && !qed)) continue;
             const bool due = c->p.sort_every > 0 && b.steps_since_sort >= c->p.sort_every;
             if ((tiled && b.n_sorted < b.n) || due) QK_TRY(sort_species<real>(c, s));
         }
@@ -325,7 +431,8 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         // (src/plist.cpp:73-102) never finds a particle here: place() keeps only
         // is_inside cells, which exclude the erased layers
         HIP_TRY(hipEventRecord(c->ev[3 * l], c->stream));
-        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        if (qed) QK_TRY(qed_particle_phase<real>(c, f, jc));
+        for (int s = 0; s < QK_MAX_SPECIES && !qed; ++s) {
             SpeciesBuf &b = c->sp[s];
             if (!b.active || b.n == 0) continue;
             PushArgs a{};
@@ -352,17 +459,20 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
             ++b.steps_since_sort;
         }
         HIP_TRY(hipEventRecord(c->ev[3 * l + 1], c->stream));
-        // compute_energy's field part sees the fields before fadvance (src/main.cpp:2327)
-        HIP_TRY(launch_field_energy<real>(c->stream, c->g, f, c->d_diag + QK_MAX_SPECIES * kDiagPerSpecies));
-        // fadvance (src/advance.cpp:13-18): B half, E, B half, interpolate_be
-        HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f));
+        // fadvance (src/advance.cpp:13-18): B half, E, B half, interpolate_be.  The
+        // first half step also sums compute_energy's field term, which sees the
+        // fields before fadvance (src/main.cpp:2327)
+        HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f,
+                                       c->d_diag + QK_MAX_SPECIES * kDiagPerSpecies));
         HIP_TRY(launch_maxwell_e<real>(c->stream, c->p.solver, c->g, c->coef, f, jc, jn));
-        HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f));
+        HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f, nullptr));
         HIP_TRY(launch_interpolate_be<real>(c->stream, c->g, f));
         HIP_TRY(hipEventRecord(c->ev[3 * l + 2], c->stream));
         c->jcur ^= 1;
         HIP_TRY(hipMemcpyAsync(hd + (size_t)l * kDiagLen, c->d_diag, kDiagLen * sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));
+        if (qed) QK_TRY(qed_download_counts(c));
+        ++c->step_index;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     float t_part = 0, t_field = 0, t_tot = 0;
@@ -455,7 +565,7 @@ int qk_create(const qk_params *params, qk_ctx **out)
     if (p.deposit != QK_DEPOSIT_TILED && p.deposit != QK_DEPOSIT_GLOBAL) return fail(QK_ERR_INVALID, "unknown deposit");
     if (p.n_ion < 0 || p.n_ion > QK_MAX_ION) return fail(QK_ERR_INVALID, "n_ion out of range");
     if (p.sort_every < 0) return fail(QK_ERR_INVALID, "sort_every must be >= 0");
-    if (p.qed) return fail(QK_ERR_NOT_BUILT, "QED is not available in this build");
+    if (p.qed && !(p.e_s > 0)) return fail(QK_ERR_INVALID, "QED needs e_s > 0");
     // the reference's deposit splits a path at no more than one face per axis
     if (p.dt >= p.dx || p.dt >= p.dy || p.dt >= p.dz)
         return fail(QK_ERR_INVALID, "dt must be below dx, dy and dz (one face crossing per axis and step)");
@@ -482,9 +592,9 @@ int qk_create(const qk_params *params, qk_ctx **out)
     c->g.dy = p.dy;
     c->g.dz = p.dz;
     c->g.dt = p.dt;
-    c->tg.ntx = (p.nx + kTile - 1) / kTile;
-    c->tg.nty = (p.ny + kTile - 1) / kTile;
-    c->tg.ntz = (p.nz + kTile - 1) / kTile;
+    c->tg.ntx = (p.nx + kTx - 1) / kTx;
+    c->tg.nty = (p.ny + kTy - 1) / kTy;
+    c->tg.ntz = (p.nz + kTz - 1) / kTz;
     c->tg.ntiles = c->tg.ntx * c->tg.nty * c->tg.ntz;
     c->tg.nkeys = c->tg.ntiles * kTileCells;
     c->rsize = p.precision == QK_FP64 ? 8 : 4;
@@ -522,6 +632,14 @@ int qk_create(const qk_params *params, qk_ctx **out)
     for (int a = 0; a < 15; ++a) c->F[a] = (char *)c->fmem + a * fbytes;
     if ((e = hipMalloc(&c->d_diag, kDiagLen * sizeof(double))) != hipSuccess)
         return bail(fail(QK_ERR_HIP, hipGetErrorString(e)));
+    if (p.qed) {
+        if ((e = hipMalloc(&c->d_count, (QK_MAX_SPECIES + 1) * sizeof(int))) != hipSuccess ||
+            (e = hipHostMalloc((void **)&c->h_count, (QK_MAX_SPECIES + 1) * sizeof(int))) != hipSuccess)
+            return bail(fail(QK_ERR_HIP, hipGetErrorString(e)));
+        // the species that QED appends to exist from the start
+        for (int s : {QK_ELECTRON, QK_POSITRON, QK_PHOTON})
+            if (int rc = ensure_species(c, s, 0)) return bail(rc);
+    }
     *out = c;
     return QK_OK;
 }
@@ -541,13 +659,16 @@ int qk_destroy(qk_ctx *c)
         if (b.chunks) note(hipFree(b.chunks));
         if (b.d_nchunks) note(hipFree(b.d_nchunks));
     }
-    int *sb[6] = {c->s_counts, c->s_offsets, c->s_tile_start, c->s_chunk_n, c->s_chunk_off, c->s_scratch};
+    int *sb[7] = {c->s_counts, c->s_offsets, c->s_tile_start, c->s_tile_mult, c->s_chunk_n, c->s_chunk_off,
+                  c->s_scratch};
     for (int *p : sb)
         if (p) note(hipFree(p));
     if (c->h_sortinfo) note(hipHostFree(c->h_sortinfo));
     if (c->fmem) note(hipFree(c->fmem));
     if (c->d_diag) note(hipFree(c->d_diag));
     if (c->h_diag) note(hipHostFree(c->h_diag));
+    if (c->d_count) note(hipFree(c->d_count));
+    if (c->h_count) note(hipHostFree(c->h_count));
     for (auto e : c->ev) note(hipEventDestroy(e));
     if (c->stream) note(hipStreamDestroy(c->stream));
     delete c;

@@ -27,11 +27,15 @@ __host__ __device__ inline int cell_i(int c) { return c >> 20; }
 __host__ __device__ inline int cell_j(int c) { return (c >> 10) & 1023; }
 __host__ __device__ inline int cell_k(int c) { return c & 1023; }
 
-// cell tiles: 8 x 8 x 8 cells, numbered i-major; particles are sorted by
-// (tile, cell inside the tile)
-constexpr int kTileShift = 3;
-constexpr int kTile = 1 << kTileShift;
-constexpr int kTileCells = kTile * kTile * kTile;
+// cell tiles of (1 << kTsx) x (1 << kTsy) x (1 << kTsz) cells, numbered
+// i-major; particles are sorted by (tile, cell inside the tile)
+#ifndef QK_TILE_SHIFTS
+#define QK_TILE_SHIFTS 3, 3, 3
+#endif
+constexpr int kTileShifts[3] = {QK_TILE_SHIFTS};
+constexpr int kTsx = kTileShifts[0], kTsy = kTileShifts[1], kTsz = kTileShifts[2];
+constexpr int kTx = 1 << kTsx, kTy = 1 << kTsy, kTz = 1 << kTsz;
+constexpr int kTileCells = kTx * kTy * kTz;
 
 struct TileGeom {
     int ntx, nty, ntz;   // tiles per axis
@@ -41,14 +45,13 @@ struct TileGeom {
 
 __host__ __device__ inline int tile_of(const TileGeom &t, int i, int j, int k)
 {
-    return ((i >> kTileShift) * t.nty + (j >> kTileShift)) * t.ntz + (k >> kTileShift);
+    return ((i >> kTsx) * t.nty + (j >> kTsy)) * t.ntz + (k >> kTsz);
 }
 
 __host__ __device__ inline int sort_key(const TileGeom &t, int c)
 {
     const int i = c >> 20, j = (c >> 10) & 1023, k = c & 1023;
-    const int m = kTile - 1;
-    return tile_of(t, i, j, k) * kTileCells + (((i & m) << (2 * kTileShift)) | ((j & m) << kTileShift) | (k & m));
+    return tile_of(t, i, j, k) * kTileCells + ((((i & (kTx - 1)) << kTsy | (j & (kTy - 1))) << kTsz) | (k & (kTz - 1)));
 }
 
 // is_inside (src/plist.cpp:42-48)
@@ -82,17 +85,43 @@ struct Species {
 
 // ---------------------------------------------------------------------------
 // gather (src/deposition.cpp:431-471): each component is bilinear in the two
-// directions transverse to it, at the lower node along its own direction
+// directions transverse to it, at the lower node along its own direction.
+// Split into its loads and its interpolation, so a kernel can issue the loads
+// of the next particle before it computes the current one.
+template <typename real>
+struct Corners {
+    real e[3][4], b[3][4];   // the four nodes each component interpolates from
+};
 
 template <typename real>
-__device__ inline void gather3(const real *__restrict__ fx, const real *__restrict__ fy,
-                               const real *__restrict__ fz, int base, int sx, int sy, real xa, real ya,
-                               real za, real out[3])
+__device__ inline void load_corners(const Fields<real> &f, int base, int sx, int sy, Corners<real> &c)
+{
+    const int ox[4] = {0, sy, 1, sy + 1};        // x components: (j, k) square
+    const int oy[4] = {0, sx, 1, sx + 1};        // y components: (i, k) square
+    const int oz[4] = {0, sx, sy, sx + sy};      // z components: (i, j) square
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        c.e[0][m] = f.ex[base + ox[m]];
+        c.e[1][m] = f.ey[base + oy[m]];
+        c.e[2][m] = f.ez[base + oz[m]];
+        c.b[0][m] = f.bex[base + ox[m]];
+        c.b[1][m] = f.bey[base + oy[m]];
+        c.b[2][m] = f.bez[base + oz[m]];
+    }
+}
+
+template <typename real>
+__device__ inline void interp_corners(const Corners<real> &c, real xa, real ya, real za, real e[3], real b[3])
 {
     const real xb = real(1) - xa, yb = real(1) - ya, zb = real(1) - za;
-    out[0] = fx[base] * yb * zb + fx[base + sy] * ya * zb + fx[base + 1] * yb * za + fx[base + sy + 1] * ya * za;
-    out[1] = fy[base] * xb * zb + fy[base + sx] * xa * zb + fy[base + 1] * xb * za + fy[base + sx + 1] * xa * za;
-    out[2] = fz[base] * xb * yb + fz[base + sx] * xa * yb + fz[base + sy] * xb * ya + fz[base + sx + sy] * xa * ya;
+    const real w[3][4] = {{yb * zb, ya * zb, yb * za, ya * za},
+                          {xb * zb, xa * zb, xb * za, xa * za},
+                          {xb * yb, xa * yb, xb * ya, xa * ya}};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        e[a] = c.e[a][0] * w[a][0] + c.e[a][1] * w[a][1] + c.e[a][2] * w[a][2] + c.e[a][3] * w[a][3];
+        b[a] = c.b[a][0] * w[a][0] + c.b[a][1] * w[a][1] + c.b[a][2] * w[a][2] + c.b[a][3] * w[a][3];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -193,84 +222,58 @@ __device__ inline void deposit_segment(Acc &acc, real q, real px, real py, real 
 // Split the straight path d, starting at p (relative to the start cell, so
 // 0 <= p < 1 on each axis), at every crossed cell face and deposit each piece
 // (jdeposition, src/deposition.cpp:41-356).  At most one face per axis is
-// crossed (|d| < 1).  With one or two crossings the pieces run through the
-// start cell, the cell with the first-crossed index switched, and the end
-// cell; ties go to the axis the reference tests first.  With three crossings
-// the fractions are sorted (regulate, src/low-level.cpp:316-366) and each
-// piece's cell is the floor of its midpoint.  The Acc policy gets a
-// set_cell(wi, wj, wk) call (offsets of the piece's cell from the start
-// cell) before each piece.  On return p holds the end point, advanced piece
-// by piece as the reference advances the particle.
+// crossed (|d| < 1), at path fraction t = (face - p) / d.  The pieces are the
+// path between consecutive crossings in increasing t (regulate,
+// src/low-level.cpp:316-366, sorts the same fractions); piece n lies in the
+// start cell with the first n crossed axes switched to their new index, which
+// is the cell the reference assigns in each of its 0-3 crossing cases (for
+// three crossings it takes the floor of the piece's midpoint, the same cell).
+// Ties between crossing fractions only reorder a zero-length piece, which
+// deposits nothing.  A wavefront runs as many pieces as its lane with the
+// most crossings, not one code path per crossing count.  The Acc policy gets
+// set_cell(wi, wj, wk) (offsets of the piece's cell from the start cell)
+// before each piece.  On return p holds the end point, advanced piece by
+// piece as the reference advances the particle.
 template <typename real, class Acc>
 __device__ inline void deposit_path(Acc &acc, real q, real p[3], const real d[3])
 {
     int c2[3];
-    c2[0] = (int)floor(p[0] + d[0]);
-    c2[1] = (int)floor(p[1] + d[1]);
-    c2[2] = (int)floor(p[2] + d[2]);
-    const int cx = c2[0] != 0, cy = c2[1] != 0, cz = c2[2] != 0;
-    const int nc = cx + cy + cz;
-    int w[3] = {0, 0, 0};
-    real s[3];
-    auto piece = [&](real f0, real f1, real f2) {
-        acc.set_cell(w[0], w[1], w[2]);
-        deposit_segment<real>(acc, q, p[0] - real(w[0]), p[1] - real(w[1]), p[2] - real(w[2]), f0, f1, f2);
-        p[0] += f0;
-        p[1] += f1;
-        p[2] += f2;
-    };
-    if (nc == 0) {
-        piece(d[0], d[1], d[2]);
-    } else if (nc == 1) {
-        const int a = cx ? 0 : (cy ? 1 : 2);
-        const real face = real(c2[a] > 0 ? c2[a] : 0);
-        const real t = face - p[a];
-        for (int m = 0; m < 3; ++m) s[m] = (m == a) ? t : d[m] * t / d[a];
-        piece(s[0], s[1], s[2]);
-        w[a] = c2[a];
-        piece(d[0] - s[0], d[1] - s[1], d[2] - s[2]);
-    } else if (nc == 2) {
-        int a0, a1;
-        if (!cx) { a0 = 1; a1 = 2; }
-        else if (!cy) { a0 = 0; a1 = 2; }
-        else { a0 = 1; a1 = 0; }
-        const real t0 = (real(c2[a0] > 0 ? c2[a0] : 0) - p[a0]) / d[a0];
-        const real t1 = (real(c2[a1] > 0 ? c2[a1] : 0) - p[a1]) / d[a1];
-        int first = a0, second = a1;
-        real tf = t0, ts = t1;
-        if (t0 > t1) { first = a1; second = a0; tf = t1; ts = t0; }
-        piece(d[0] * tf, d[1] * tf, d[2] * tf);
-        w[first] = c2[first];
-        const real tm = ts - tf;
-        piece(d[0] * tm, d[1] * tm, d[2] * tm);
-        w[second] = c2[second];
-        const real te = real(1) - ts;
-        piece(d[0] * te, d[1] * te, d[2] * te);
-    } else {
-        real t[3];
-        for (int m = 0; m < 3; ++m) t[m] = (real(c2[m] > 0 ? c2[m] : 0) - p[m]) / d[m];
-        // descending order as regulate() returns it
-        real v0, v1, v2;
-        if (t[0] > t[1]) {
-            if (t[1] > t[2]) { v0 = t[0]; v1 = t[1]; v2 = t[2]; }
-            else if (t[0] > t[2]) { v0 = t[0]; v1 = t[2]; v2 = t[1]; }
-            else { v0 = t[2]; v1 = t[0]; v2 = t[1]; }
-        } else {
-            if (t[2] > t[1]) { v0 = t[2]; v1 = t[1]; v2 = t[0]; }
-            else if (t[0] > t[2]) { v0 = t[1]; v1 = t[0]; v2 = t[2]; }
-            else { v0 = t[1]; v1 = t[2]; v2 = t[0]; }
+    real t[3];
+    int ax[3] = {0, 1, 2};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        c2[m] = (int)floor(p[m] + d[m]);
+        // non-crossing axes sort after every crossing one
+        t[m] = c2[m] != 0 ? (real(c2[m] > 0 ? 1 : 0) - p[m]) * qk_rcp(d[m]) : real(2);
+    }
+    const int nc = (c2[0] != 0) + (c2[1] != 0) + (c2[2] != 0);
+    auto cswap = [&](int a, int b) {
+        if (t[a] > t[b]) {
+            const real tt = t[a]; t[a] = t[b]; t[b] = tt;
+            const int ta = ax[a]; ax[a] = ax[b]; ax[b] = ta;
         }
-        const real fr[4] = {v2, v1 - v2, v0 - v1, real(1) - v0};
-        for (int n = 0; n < 4; ++n) {
-            s[0] = d[0] * fr[n];
-            s[1] = d[1] * fr[n];
-            s[2] = d[2] * fr[n];
-            if (n > 0) {
-                w[0] = (int)floor(p[0] + real(0.5) * s[0]);
-                w[1] = (int)floor(p[1] + real(0.5) * s[1]);
-                w[2] = (int)floor(p[2] + real(0.5) * s[2]);
-            }
-            piece(s[0], s[1], s[2]);
+    };
+    cswap(0, 1);
+    cswap(1, 2);
+    cswap(0, 1);
+    int w[3] = {0, 0, 0};
+    real done = real(0);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        if (n > nc) break;
+        const real upto = n < nc ? t[n] : real(1);
+        const real fr = upto - done;
+        done = upto;
+        const real s0 = d[0] * fr, s1 = d[1] * fr, s2 = d[2] * fr;
+        acc.set_cell(w[0], w[1], w[2]);
+        deposit_segment<real>(acc, q, p[0] - real(w[0]), p[1] - real(w[1]), p[2] - real(w[2]), s0, s1, s2);
+        p[0] += s0;
+        p[1] += s1;
+        p[2] += s2;
+        if (n < 3) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                if (ax[n] == m) w[m] = c2[m];
         }
     }
 }

@@ -111,25 +111,48 @@ __device__ inline real diff(const Grid &g, const Coef<real> &cf, const real *__r
     }
 }
 
-// half step of B (maxwell_solver::advance_b, src/maxwell.cpp:10-22)
+// half step of B (maxwell_solver::advance_b, src/maxwell.cpp:10-22).  With
+// energy != null the first half step also sums E^2 + B^2 of the fields it
+// starts from, in FP64: compute_energy's field term, which the reference takes
+// between padvance and fadvance (src/fout.cpp:102, src/main.cpp:2327-2331).
 template <int S, typename real>
 __global__ void __launch_bounds__(kBX * kBY) k_maxwell_b(Grid g, Coef<real> cf, const real *__restrict__ ex,
                                                          const real *__restrict__ ey, const real *__restrict__ ez,
                                                          real *__restrict__ bx, real *__restrict__ by,
-                                                         real *__restrict__ bz)
+                                                         real *__restrict__ bz, double *energy)
 {
+    __shared__ double part[kBX * kBY / 64];
     const int k = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
     const int i = blockIdx.z;
-    if (k >= g.nz || j >= g.ny) return;
-    const int id = i * g.sx + j * g.sy + k;
-    const real h = real(0.5);
-    if (b_updates<S, 0>(g, i, j, k))
-        bx[id] += h * (-cf.dtd[1] * diff<S, 2, 1, true>(g, cf, ez, i, j, k) + cf.dtd[2] * diff<S, 1, 2, true>(g, cf, ey, i, j, k));
-    if (b_updates<S, 1>(g, i, j, k))
-        by[id] += h * (cf.dtd[0] * diff<S, 2, 0, true>(g, cf, ez, i, j, k) - cf.dtd[2] * diff<S, 0, 2, true>(g, cf, ex, i, j, k));
-    if (b_updates<S, 2>(g, i, j, k))
-        bz[id] += h * (cf.dtd[1] * diff<S, 0, 1, true>(g, cf, ex, i, j, k) - cf.dtd[0] * diff<S, 1, 0, true>(g, cf, ey, i, j, k));
+    double en = 0;
+    if (k < g.nz && j < g.ny) {
+        const int id = i * g.sx + j * g.sy + k;
+        const real h = real(0.5);
+        const real b0 = bx[id], b1 = by[id], b2 = bz[id];
+        if (energy) {
+            const double e0 = ex[id], e1 = ey[id], e2 = ez[id];
+            en = e0 * e0 + e1 * e1 + e2 * e2 + (double)b0 * b0 + (double)b1 * b1 + (double)b2 * b2;
+        }
+        if (b_updates<S, 0>(g, i, j, k))
+            bx[id] = b0 + h * (-cf.dtd[1] * diff<S, 2, 1, true>(g, cf, ez, i, j, k) + cf.dtd[2] * diff<S, 1, 2, true>(g, cf, ey, i, j, k));
+        if (b_updates<S, 1>(g, i, j, k))
+            by[id] = b1 + h * (cf.dtd[0] * diff<S, 2, 0, true>(g, cf, ez, i, j, k) - cf.dtd[2] * diff<S, 0, 2, true>(g, cf, ex, i, j, k));
+        if (b_updates<S, 2>(g, i, j, k))
+            bz[id] = b2 + h * (cf.dtd[1] * diff<S, 0, 1, true>(g, cf, ex, i, j, k) - cf.dtd[0] * diff<S, 1, 0, true>(g, cf, ey, i, j, k));
+    }
+    if (energy) {   // uniform across the launch
+        en = wave_sum(en);
+        const int t = threadIdx.y * kBX + threadIdx.x;
+        if ((t & 63) == 0) part[t >> 6] = en;
+        __syncthreads();
+        if (t == 0) {
+            double s = 0;
+            for (int w = 0; w < kBX * kBY / 64; ++w) s += part[w];
+            const int slot = (int)((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) % kEnergySlots;
+            atomicAdd(energy + slot, s);
+        }
+    }
 }
 
 // E step with the current of this step; also clears the other current
@@ -245,23 +268,6 @@ __global__ void __launch_bounds__(kBX * kBY) k_init_boundaries(Grid g, int solve
     if (bm & 4) f.bz[id] = real(0);
 }
 
-// sum of E^2 + B^2 over all cells in FP64 (compute_energy, src/fout.cpp:102)
-template <typename real>
-__global__ void __launch_bounds__(256) k_field_energy(int n, Fields<real> f, double *acc)
-{
-    __shared__ double part[4];
-    double s = 0;
-    for (int id = blockIdx.x * 256 + threadIdx.x; id < n; id += gridDim.x * 256) {
-        const double a = f.ex[id], b = f.ey[id], c = f.ez[id];
-        const double d = f.bx[id], e = f.by[id], h = f.bz[id];
-        s += a * a + b * b + c * c + d * d + e * e + h * h;
-    }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
-}
-
 // Laser initial condition f_init_cos with the cos envelope (src/init.cpp:6-210),
 // evaluated in FP64 and stored in the field precision.  L: a0y a0z xsigma
 // ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt
@@ -344,13 +350,14 @@ hipError_t launch_zero(hipStream_t st, double *p, size_t n)
 }
 
 template <typename real>
-hipError_t launch_maxwell_b(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f)
+hipError_t launch_maxwell_b(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f,
+                            double *energy)
 {
     const Coef<real> cf = make_coef<real>(g, c);
     const dim3 grid = cell_grid(g), block(kBX, kBY);
-    if (solver == 0) k_maxwell_b<0, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz);
-    else if (solver == 1) k_maxwell_b<1, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz);
-    else k_maxwell_b<2, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz);
+    if (solver == 0) k_maxwell_b<0, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz, energy);
+    else if (solver == 1) k_maxwell_b<1, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz, energy);
+    else k_maxwell_b<2, real><<<grid, block, 0, st>>>(g, cf, f.ex, f.ey, f.ez, f.bx, f.by, f.bz, energy);
     return hipGetLastError();
 }
 
@@ -379,15 +386,6 @@ template <typename real>
 hipError_t launch_init_boundaries(hipStream_t st, int solver, const Grid &g, const Fields<real> &f)
 {
     k_init_boundaries<real><<<cell_grid(g), dim3(kBX, kBY), 0, st>>>(g, solver, f);
-    return hipGetLastError();
-}
-
-template <typename real>
-hipError_t launch_field_energy(hipStream_t st, const Grid &g, const Fields<real> &f, double *acc)
-{
-    int blocks = (g.ncell + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    k_field_energy<real><<<blocks, 256, 0, st>>>(g.ncell, f, acc);
     return hipGetLastError();
 }
 
@@ -437,12 +435,12 @@ hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f
 }
 
 #define QK_INST(real)                                                                                            \
-    template hipError_t launch_maxwell_b<real>(hipStream_t, int, const Grid &, const SolverCoef &, const Fields<real> &); \
+    template hipError_t launch_maxwell_b<real>(hipStream_t, int, const Grid &, const SolverCoef &, const Fields<real> &, \
+                                               double *);                                                       \
     template hipError_t launch_maxwell_e<real>(hipStream_t, int, const Grid &, const SolverCoef &, const Fields<real> &, \
                                                const Current<real> &, const Current<real> &);                   \
     template hipError_t launch_interpolate_be<real>(hipStream_t, const Grid &, const Fields<real> &);           \
     template hipError_t launch_init_boundaries<real>(hipStream_t, int, const Grid &, const Fields<real> &);     \
-    template hipError_t launch_field_energy<real>(hipStream_t, const Grid &, const Fields<real> &, double *);   \
     template hipError_t launch_laser_cos<real>(hipStream_t, const Grid &, const Fields<real> &, const double *);
 QK_INST(float)
 QK_INST(double)

@@ -52,8 +52,53 @@ template <typename real>
 hipError_t launch_push_tiled(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
                              const Species<real> &s, const PushArgs &a);
 
+// ---- QED -------------------------------------------------------------------
+
+// constants of one QED step
+struct QedArgs {
+    double e_s;          // Sauter field (mc omega / e units)
+    uint64_t seed;       // run seed of the counter-based streams
+    uint32_t step;       // global step index (counter word)
+};
+
+// a species buffer that kernels append new particles to: slot = atomic
+// increment of *count; an append at or beyond cap is dropped and sets
+// *overflow (the host reserves enough capacity before each step)
 template <typename real>
-hipError_t launch_maxwell_b(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f);
+struct AppendTarget {
+    Species<real> s;
+    int *count;
+    int cap;
+    int *overflow;
+    double *diag;        // kDiagPerSpecies accumulators of the target species
+};
+
+// electrons / positrons with photon emission (src/advance.cpp:253-359);
+// every current contribution a global atomic; emitted photons go to ph
+template <typename real>
+hipError_t launch_push_qed_lepton(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
+                                  const Species<real> &s, const PushArgs &a, const QedArgs &q,
+                                  const AppendTarget<real> &ph);
+// photons with Breit-Wheeler pair creation (src/advance.cpp:155-252); the
+// pairs go to el / po
+template <typename real>
+hipError_t launch_photon_qed(hipStream_t st, const Grid &g, const Fields<real> &f, const Current<real> &j,
+                             const Species<real> &s, const PushArgs &a, const QedArgs &q,
+                             const AppendTarget<real> &el, const AppendTarget<real> &po);
+// birth_from_vacuum (src/mc.cpp:142-196): one draw per cell 2 .. n-3
+template <typename real>
+hipError_t launch_vacuum_pairs(hipStream_t st, const Grid &g, const Fields<real> &f, int pusher, double q_vac,
+                               const QedArgs &q, const AppendTarget<real> &el, const AppendTarget<real> &po);
+
+// the field-energy sum is spread over this many FP64 slots (one atomic per
+// workgroup lands on slot blockIdx mod kEnergySlots) and summed on the host
+constexpr int kEnergySlots = 256;
+
+// half step of B; with energy != null also accumulates sum(E^2 + B^2) of the
+// incoming fields into energy[0 .. kEnergySlots) (FP64)
+template <typename real>
+hipError_t launch_maxwell_b(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f,
+                            double *energy);
 template <typename real>
 hipError_t launch_maxwell_e(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f,
                             const Current<real> &j, const Current<real> &j_next);
@@ -61,8 +106,6 @@ template <typename real>
 hipError_t launch_interpolate_be(hipStream_t st, const Grid &g, const Fields<real> &f);
 template <typename real>
 hipError_t launch_init_boundaries(hipStream_t st, int solver, const Grid &g, const Fields<real> &f);
-template <typename real>
-hipError_t launch_field_energy(hipStream_t st, const Grid &g, const Fields<real> &f, double *acc);
 template <typename real>
 hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f, const double *L);
 template <typename real>
@@ -82,6 +125,7 @@ struct SortBuffers {
     int *counts;      // nkeys + 1
     int *offsets;     // nkeys + 1 (consumed as cursors by the scatter)
     int *tile_start;  // ntiles + 1
+    int *tile_mult;   // ntiles: inverse spread multiplier of the tile's storage order
     int *chunk_n;     // ntiles + 1
     int *chunk_off;   // ntiles + 1
     int4 *chunks;     // max_chunks
@@ -89,7 +133,9 @@ struct SortBuffers {
     int max_chunks;
 };
 
-// sort the live particles of src[0, n) into dst by (tile, cell); writes
+// sort the live particles of src[0, n) into dst by tile, each tile's
+// particles in its spread storage order (consecutive slots hold particles
+// of distinct cells; see k_tile_chunks); writes
 // tile_start, the chunk work list and its length (chunk_off[ntiles])
 template <typename real>
 hipError_t launch_sort(hipStream_t st, const TileGeom &tg, const Species<real> &src, const Species<real> &dst,

@@ -6,178 +6,130 @@
 //   k_push_global  one thread per particle, each current contribution a global
 //                  atomic (the plain method, used for the FP64 comparison)
 //   k_push_tiled   one workgroup per chunk of an 8x8x8 cell tile; particles are
-//                  sorted by (tile, cell), so the wavefront's gathers hit a few
-//                  cache lines and its deposits land in an LDS window of the
-//                  tile plus a halo, flushed once with one global atomic per
-//                  nonzero window entry.  A particle that drifted out of the
-//                  window since the last sort deposits through global atomics.
-#include "qk_kernels.h"
+//                  sorted by tile, so the workgroup stages the tile's E and Be
+//                  nodes in LDS once, gathers from there, and deposits into an
+//                  LDS window of the tile plus a halo, flushed once with one
+//                  global atomic per nonzero window entry.  Inside a tile the
+//                  storage order puts particles of distinct cells in
+//                  consecutive slots, so a wavefront's LDS atomics do not
+//                  collide.  A particle that drifted out of the window since
+//                  the last sort gathers and deposits through global memory.
+#include "qk_push.h"
 
 namespace qk {
 
 namespace {
 
-constexpr int kBlock = 256;
-// the window covers start cells within kHalo cells of the tile; a deposit
-// reaches one cell below and two above the start cell
-constexpr int kHalo = 1;
-constexpr int kWin = kTile + 2 * kHalo + 3;
-constexpr int kWin3 = kWin * kWin * kWin;
+// A workgroup of the tiled kernel serves particles whose start cell lies
+// within kHalo cells of its tile ("in the window").  Relative to the tile's
+// lower corner such a cell has r in [-kHalo, kT + kHalo) on each axis.
+#ifndef QK_HALO
+#define QK_HALO 1
+#endif
+constexpr int kHalo = QK_HALO;
+constexpr int kT[3] = {kTx, kTy, kTz};
 
-template <typename real>
-struct StepConst {
-    real cmr_dt, ext_bz, inv_d[3], dt;
-    double inv_cmr;
-    int energy_mode;
-};
+// Current window, FP64, one block per component.  A deposit from start cell
+// r touches cells r-1 .. r+1 (one face crossing per axis at most) and, for
+// the two axes transverse to the component, the node above as well: the
+// window spans r - 1 .. r + 1 along the component's axis and r - 1 .. r + 2
+// across it, i.e. window index = r + kHalo + 1.
+__host__ __device__ constexpr int jdim(int c, int a) { return kT[a] + 2 * kHalo + (a == c ? 2 : 3); }
+__host__ __device__ constexpr int jsize(int c) { return jdim(c, 0) * jdim(c, 1) * jdim(c, 2); }
+__host__ __device__ constexpr int joff(int c) { return c == 0 ? 0 : (c == 1 ? jsize(0) : jsize(0) + jsize(1)); }
+constexpr int kJWin = jsize(0) + jsize(1) + jsize(2);
 
-// every contribution is a global atomic
-template <typename real>
-struct GlobalAcc {
-    real *jx, *jy, *jz;
-    int sx, sy;
-    int base;   // linear index of the particle's start cell
-    int cur;    // linear index of the current piece's cell
-    __device__ inline void set_cell(int wi, int wj, int wk) { cur = base + wi * sx + wj * sy + wk; }
-    __device__ inline void operator()(int comp, int di, int dj, int dk, real v)
-    {
-        const int id = cur + di * sx + dj * sy + dk;
-        real *a = comp == 0 ? jx : (comp == 1 ? jy : jz);
-        unsafeAtomicAdd(a + id, v);
-    }
-};
+// Field stage: E and Be nodes a window particle interpolates from.  Each
+// component needs its own axis at the start cell only and the node above
+// across it (gather at the lower node, src/deposition.cpp:431-471): index =
+// r + kHalo, extent kT + 2 kHalo along the component, one more across.
+__host__ __device__ constexpr int fdim(int c, int a) { return kT[a] + 2 * kHalo + (a == c ? 0 : 1); }
+__host__ __device__ constexpr int fsize(int c) { return fdim(c, 0) * fdim(c, 1) * fdim(c, 2); }
+__host__ __device__ constexpr int foff(int c) { return c == 0 ? 0 : (c == 1 ? fsize(0) : fsize(0) + fsize(1)); }
+constexpr int kFStage = fsize(0) + fsize(1) + fsize(2);   // per field (E or Be)
 
-// contributions go to the workgroup's LDS window [3][kWin][kWin][kWin].  The
-// window is FP64 for both particle precisions: on gfx950 an FP64 LDS atomic
-// add sustains about twenty times the rate of an FP32 one, and the tile's
-// partial sums keep more bits before the single FP32/FP64 flush.
+// contributions go to the workgroup's LDS current window (one block per
+// component, dims jdim).  The window is FP64 for both particle precisions:
+// on gfx950 an FP64 LDS atomic add sustains about twenty times the rate of
+// an FP32 one, and the tile's partial sums keep more bits before the single
+// FP32/FP64 flush.
 template <typename real>
 struct LdsAcc {
     double *win;
-    int base;   // window index of the particle's start cell
-    int cur;
-    __device__ inline void set_cell(int wi, int wj, int wk) { cur = base + (wi * kWin + wj) * kWin + wk; }
+    int base[3];   // window index of the particle's start cell, per component
+    int cur[3];    // ... of the current piece's cell
+    __device__ inline LdsAcc(double *w, int ri, int rj, int rk) : win(w)
+    {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            base[c] = ((ri + kHalo + 1) * jdim(c, 1) + (rj + kHalo + 1)) * jdim(c, 2) + (rk + kHalo + 1);
+    }
+    __device__ inline void set_cell(int wi, int wj, int wk)
+    {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cur[c] = base[c] + (wi * jdim(c, 1) + wj) * jdim(c, 2) + wk;
+    }
     __device__ inline void operator()(int comp, int di, int dj, int dk, real v)
     {
-        atomicAdd(win + comp * kWin3 + cur + (di * kWin + dj) * kWin + dk, (double)v);
+        const int d1 = jdim(comp, 1), d2 = jdim(comp, 2);
+        atomicAdd(win + joff(comp) + cur[comp] + (di * d1 + dj) * d2 + dk, (double)v);
     }
 };
 
-__device__ inline void block_accumulate(double *dst, double v0, double v1, double v2, double v3)
-{
-    __shared__ double part[4][kBlock / 64];
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    v2 = wave_sum(v2);
-    v3 = wave_sum(v3);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        part[0][w] = v0;
-        part[1][w] = v1;
-        part[2][w] = v2;
-        part[3][w] = v3;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s[4] = {0, 0, 0, 0};
-        for (int m = 0; m < kBlock / 64; ++m)
-            for (int a = 0; a < 4; ++a) s[a] += part[a][m];
-        if (s[2] != 0 || s[3] != 0)
-            for (int a = 0; a < 4; ++a) atomicAdd(dst + a, s[a]);
-    }
-}
-
-struct DiagAcc {
-    double q = 0, e = 0, n = 0, fallback = 0;
-};
-
-// one particle in registers
+// Corner nodes of a window particle from the field stage (E then Be, one
+// block per component, dims fdim); the same squares as load_corners.
 template <typename real>
-struct PRec {
-    int cell;      // packed cell, negative: free slot
-    real o[3];     // offset in the cell
-    real u[3];     // u = p / m c
-    real q;        // macro charge
-};
-
-template <typename real>
-__device__ inline void load_rec(const Species<real> &s, int64_t p, PRec<real> &r)
+__device__ inline void load_corners_stage(const real *st, int ri, int rj, int rk, Corners<real> &c)
 {
-    r.cell = s.cell[p];
-    r.o[0] = s.ox[p]; r.o[1] = s.oy[p]; r.o[2] = s.oz[p];
-    r.u[0] = s.ux[p]; r.u[1] = s.uy[p]; r.u[2] = s.uz[p];
-    r.q = s.q[p];
-}
-
-// write back what a push changes (the charge does not change)
-template <typename real>
-__device__ inline void store_rec(const Species<real> &s, int64_t p, const PRec<real> &r)
-{
-    s.cell[p] = r.cell;
-    s.ox[p] = r.o[0]; s.oy[p] = r.o[1]; s.oz[p] = r.o[2];
-    s.ux[p] = r.u[0]; s.uy[p] = r.u[1]; s.uz[p] = r.u[2];
-}
-
-// One charged particle with QED off: gather E and Be at the particle, add
-// external Bz, push (Vay or Boris), deposit the current of the straight path
-// u dt / gamma split at cell faces, move, and mark the slot free if the new
-// cell is not is_inside (place(), src/plist.cpp:7-40).  Diagnostics of
-// compute_N / compute_energy (src/fout.cpp:41-138) come from the post-push
-// state.  dep(i, j, k, base, q, pos, d, da) performs the deposit and
-// advances pos.
-template <typename real, int PUSHER, class Dep>
-__device__ inline void advance_particle(const Grid &g, const Fields<real> &f, PRec<real> &r, const StepConst<real> &sc,
-                                        Dep &&dep, DiagAcc &da)
-{
-    const int c = r.cell;
-    if (c < 0) return;
-    const int i = cell_i(c), j = cell_j(c), k = cell_k(c);
-    // local copies stay in registers (the deposit indexes them by axis)
-    real pos[3] = {r.o[0], r.o[1], r.o[2]};
-    real u[3] = {r.u[0], r.u[1], r.u[2]};
-    const int base = i * g.sx + j * g.sy + k;
-    real e[3], b[3];
-    gather3(f.ex, f.ey, f.ez, base, g.sx, g.sy, pos[0], pos[1], pos[2], e);
-    gather3(f.bex, f.bey, f.bez, base, g.sx, g.sy, pos[0], pos[1], pos[2], b);
-    b[2] += sc.ext_bz;
-    real gam = qk_sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    push<PUSHER>(sc.cmr_dt, e, b, u, gam);
-    // displacement in cells (containers.cpp:58-67, advance.cpp:281-284)
-    const real a = sc.dt * qk_rcp(gam);
-    const real d[3] = {u[0] * a * sc.inv_d[0], u[1] * a * sc.inv_d[1], u[2] * a * sc.inv_d[2]};
-    // |d| < dt/dx < 1 for any finite momentum; a non-finite one (corrupt
-    // input) would index outside the stencil, so that particle is dropped
-    r.u[0] = u[0];
-    r.u[1] = u[1];
-    r.u[2] = u[2];
-    if (!(fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1))) {
-        r.cell = -1;
-        return;
-    }
-    dep(i, j, k, base, r.q, pos, d, da);
-    int ci[3];
+    int b[3];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        const real fl = floor(pos[m]);
-        ci[m] = (int)fl;
-        pos[m] -= fl;
-        if (pos[m] >= real(1)) {   // rounding of a tiny negative position
-            pos[m] = real(0);
-            ci[m] += 1;
-        }
+    for (int a = 0; a < 3; ++a)
+        b[a] = foff(a) + ((ri + kHalo) * fdim(a, 1) + (rj + kHalo)) * fdim(a, 2) + (rk + kHalo);
+    const int sx0 = fdim(0, 2), sx1 = fdim(1, 1) * fdim(1, 2), sy2 = fdim(2, 1) * fdim(2, 2), sz2 = fdim(2, 2);
+    const int ox[4] = {0, sx0, 1, sx0 + 1};          // x components: (j, k) square
+    const int oy[4] = {0, sx1, 1, sx1 + 1};          // y components: (i, k) square
+    const int oz[4] = {0, sy2, sz2, sy2 + sz2};      // z components: (i, j) square
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        c.e[0][m] = st[b[0] + ox[m]];
+        c.e[1][m] = st[b[1] + oy[m]];
+        c.e[2][m] = st[b[2] + oz[m]];
+        c.b[0][m] = st[kFStage + b[0] + ox[m]];
+        c.b[1][m] = st[kFStage + b[1] + oy[m]];
+        c.b[2][m] = st[kFStage + b[2] + oz[m]];
     }
-    r.o[0] = pos[0];
-    r.o[1] = pos[1];
-    r.o[2] = pos[2];
-    const int ni = i + ci[0], nj = j + ci[1], nk = k + ci[2];
-    if (is_inside(g, ni, nj, nk)) {
-        r.cell = pack_cell(ni, nj, nk);
-        const double gg = (double)gam;
-        da.q += (double)r.q;
-        da.n += 1;
-        da.e += sc.energy_mode == 2 ? (gg - 1) * (double)r.q * sc.inv_cmr : (gg - 1) * (double)r.q;
-    } else {
-        r.cell = -1;
+}
+
+// copy component C of one field into its stage block; nodes outside the
+// grid (never read by a particle) are set to zero
+template <int C, typename real>
+__device__ inline void stage_component(real *dst, const real *__restrict__ src, const Grid &g, int gi0, int gj0,
+                                       int gk0)
+{
+    constexpr int d1 = fdim(C, 1), d2 = fdim(C, 2);
+    for (int w = threadIdx.x; w < fsize(C); w += kBlock) {
+        const int lk = w % d2, t = w / d2;
+        const int lj = t % d1, li = t / d1;
+        const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
+        const bool in = (unsigned)gi < (unsigned)g.nx && (unsigned)gj < (unsigned)g.ny && (unsigned)gk < (unsigned)g.nz;
+        dst[foff(C) + w] = in ? src[gi * g.sx + gj * g.sy + gk] : real(0);
+    }
+}
+
+// add component C of the current window to the grid: one global atomic per
+// nonzero entry inside the grid
+template <int C, typename real>
+__device__ inline void flush_component(real *__restrict__ dst, const double *win, const Grid &g, int gi0, int gj0,
+                                       int gk0)
+{
+    constexpr int d1 = jdim(C, 1), d2 = jdim(C, 2);
+    for (int w = threadIdx.x; w < jsize(C); w += kBlock) {
+        const int wk = w % d2, t = w / d2;
+        const int wj = t % d1, wi = t / d1;
+        const int gi = gi0 + wi, gj = gj0 + wj, gk = gk0 + wk;
+        const double v = win[joff(C) + w];
+        if (v != 0.0 && (unsigned)gi < (unsigned)g.nx && (unsigned)gj < (unsigned)g.ny && (unsigned)gk < (unsigned)g.nz)
+            unsafeAtomicAdd(dst + gi * g.sx + gj * g.sy + gk, (real)v);
     }
 }
 
@@ -195,42 +147,48 @@ __global__ void __launch_bounds__(kBlock) k_push_global(Grid g, Fields<real> f, 
         PRec<real> r;
         load_rec(s, p, r);
         if (r.cell >= 0) {
-            advance_particle<real, PUSHER>(g, f, r, sc, dep, da);
+            Corners<real> nodes;
+            load_corners(f, cell_base(g, r.cell), g.sx, g.sy, nodes);
+            advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
             store_rec(s, p, r);
         }
     }
     block_accumulate(diag, da.q, da.e, da.n, da.fallback);
 }
 
-// Workgroup order inside a block of kBlock particles: lane t works on
-// particle (kPerm * t) mod kBlock of the block.  Sorted particles share a cell
-// in runs of about ppc; the odd stride spreads a wavefront over distinct cells
-// (so its LDS atomics hit distinct addresses) and distinct LDS banks, while
-// the workgroup as a whole still reads and writes whole cache lines.
-constexpr int kPerm = 9;
 
 template <typename real, int PUSHER>
 __global__ void __launch_bounds__(kBlock) k_push_tiled(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
                                                        StepConst<real> sc, const int4 *__restrict__ chunks,
                                                        const int *__restrict__ nchunks, TileGeom tg, double *diag)
 {
-    __shared__ double win[3 * kWin3];
+    __shared__ double win[kJWin];
+    __shared__ real stage[2 * kFStage];
     if ((int)blockIdx.x >= *nchunks) return;   // whole workgroup leaves together
     const int4 ch = chunks[blockIdx.x];
     const int tile = ch.x;
     const int tk = tile % tg.ntz, t2 = tile / tg.ntz;
     const int tj = t2 % tg.nty, ti = t2 / tg.nty;
-    const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
-    for (int w = threadIdx.x; w < 3 * kWin3; w += kBlock) win[w] = 0.0;
+    const int i0 = ti * kTx, j0 = tj * kTy, k0 = tk * kTz;
+    for (int w = threadIdx.x; w < kJWin; w += kBlock) win[w] = 0.0;
+    {
+        const int si = i0 - kHalo, sj = j0 - kHalo, sk = k0 - kHalo;
+        stage_component<0>(stage, f.ex, g, si, sj, sk);
+        stage_component<1>(stage, f.ey, g, si, sj, sk);
+        stage_component<2>(stage, f.ez, g, si, sj, sk);
+        stage_component<0>(stage + kFStage, f.bex, g, si, sj, sk);
+        stage_component<1>(stage + kFStage, f.bey, g, si, sj, sk);
+        stage_component<2>(stage + kFStage, f.bez, g, si, sj, sk);
+    }
     __syncthreads();
 
     DiagAcc da;
-    auto dep = [&](int i, int j, int k, int base, real q, real pos[3], const real d[3], DiagAcc &acc_d) {
-        const int ri = i - i0 + kHalo, rj = j - j0 + kHalo, rk = k - k0 + kHalo;
-        const int lim = kTile + 2 * kHalo;
-        if ((unsigned)ri < (unsigned)lim && (unsigned)rj < (unsigned)lim && (unsigned)rk < (unsigned)lim) {
-            // window index of the start cell: one extra layer below for w = -1
-            LdsAcc<real> acc{win, ((ri + 1) * kWin + (rj + 1)) * kWin + (rk + 1), 0};
+    // start cell relative to the tile, and whether it lies in the window
+    int ri = 0, rj = 0, rk = 0;
+    bool in_win = false;
+    auto dep = [&](int, int, int, int base, real q, real pos[3], const real d[3], DiagAcc &acc_d) {
+        if (in_win) {
+            LdsAcc<real> acc(win, ri, rj, rk);
             deposit_path<real>(acc, q, pos, d);
         } else {
             GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
@@ -239,32 +197,42 @@ __global__ void __launch_bounds__(kBlock) k_push_tiled(Grid g, Fields<real> f, C
         }
     };
     const int64_t end = (int64_t)ch.y + ch.z;
-    const int sidx = (kPerm * (int)threadIdx.x) & (kBlock - 1);
-    for (int64_t b0 = ch.y; b0 < end; b0 += kBlock) {
-        const int64_t p = b0 + sidx;
-        if (p < end) {
-            PRec<real> r;
+#ifndef QK_PREFETCH
+#define QK_PREFETCH 1
+#endif
+    // the next record is loaded while the current one is processed
+    PRec<real> nxt;
+    int64_t p = (int64_t)ch.y + threadIdx.x;
+    if (QK_PREFETCH && p < end) load_rec(s, p, nxt);
+    for (; p < end; p += kBlock) {
+        PRec<real> r;
+        if (QK_PREFETCH) {
+            r = nxt;
+            if (p + kBlock < end) load_rec(s, p + kBlock, nxt);
+        } else {
             load_rec(s, p, r);
-            if (r.cell >= 0) {
-                advance_particle<real, PUSHER>(g, f, r, sc, dep, da);
-                store_rec(s, p, r);
-            }
+        }
+        if (r.cell >= 0) {
+            ri = cell_i(r.cell) - i0;
+            rj = cell_j(r.cell) - j0;
+            rk = cell_k(r.cell) - k0;
+            in_win = (unsigned)(ri + kHalo) < (unsigned)(kTx + 2 * kHalo) &&
+                     (unsigned)(rj + kHalo) < (unsigned)(kTy + 2 * kHalo) &&
+                     (unsigned)(rk + kHalo) < (unsigned)(kTz + 2 * kHalo);
+            Corners<real> nodes;
+            if (in_win) load_corners_stage(stage, ri, rj, rk, nodes);
+            else load_corners(f, cell_base(g, r.cell), g.sx, g.sy, nodes);
+            advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
+            store_rec(s, p, r);
         }
     }
     __syncthreads();
 
-    // flush the window: one global atomic per nonzero entry inside the grid
-    const int wi0 = i0 - kHalo - 1, wj0 = j0 - kHalo - 1, wk0 = k0 - kHalo - 1;
-    for (int w = threadIdx.x; w < kWin3; w += kBlock) {
-        const int wk = w % kWin, t = w / kWin;
-        const int wj = t % kWin, wi = t / kWin;
-        const int gi = wi0 + wi, gj = wj0 + wj, gk = wk0 + wk;
-        if ((unsigned)gi >= (unsigned)g.nx || (unsigned)gj >= (unsigned)g.ny || (unsigned)gk >= (unsigned)g.nz) continue;
-        const int id = gi * g.sx + gj * g.sy + gk;
-        const real vx = win[w], vy = win[kWin3 + w], vz = win[2 * kWin3 + w];
-        if (vx != real(0)) unsafeAtomicAdd(cur.jx + id, vx);
-        if (vy != real(0)) unsafeAtomicAdd(cur.jy + id, vy);
-        if (vz != real(0)) unsafeAtomicAdd(cur.jz + id, vz);
+    {
+        const int wi0 = i0 - kHalo - 1, wj0 = j0 - kHalo - 1, wk0 = k0 - kHalo - 1;
+        flush_component<0>(cur.jx, win, g, wi0, wj0, wk0);
+        flush_component<1>(cur.jy, win, g, wi0, wj0, wk0);
+        flush_component<2>(cur.jz, win, g, wi0, wj0, wk0);
     }
     block_accumulate(diag, da.q, da.e, da.n, da.fallback);
 }
@@ -376,23 +344,6 @@ __global__ void __launch_bounds__(kBlock) k_compute_rho(Grid g, Species<real> s,
     unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sx + 1), q * xa * yb * za);
     unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sy + 1), q * xb * ya * za);
     unsafeAtomicAdd(r + 3 * (int64_t)(b + g.sx + g.sy + 1), q * xa * ya * za);
-}
-
-inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
-
-template <typename real>
-StepConst<real> step_const(const Grid &g, const PushArgs &a)
-{
-    StepConst<real> sc;
-    sc.cmr_dt = (real)(a.cmr * g.dt);
-    sc.ext_bz = (real)a.external_bz;
-    sc.inv_d[0] = (real)(1.0 / g.dx);
-    sc.inv_d[1] = (real)(1.0 / g.dy);
-    sc.inv_d[2] = (real)(1.0 / g.dz);
-    sc.dt = (real)g.dt;
-    sc.inv_cmr = a.cmr != 0 ? 1.0 / a.cmr : 0.0;
-    sc.energy_mode = a.energy_mode;
-    return sc;
 }
 
 }  // namespace

@@ -97,7 +97,8 @@ __global__ void __launch_bounds__(256) k_count_keys(const int *__restrict__ cell
 
 template <typename real>
 __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real> dst, int64_t n, TileGeom tg,
-                                                 int *cursor)
+                                                 int *cursor, const int *__restrict__ tile_start,
+                                                 const int *__restrict__ tile_mult)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int c = p < n ? src.cell[p] : -1;
@@ -109,7 +110,10 @@ __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real
     if (lane == head && key >= 0) base = atomicAdd(cursor + key, len);
     base = __shfl(base, head, 64);
     if (c < 0) return;
-    const int o = base + (lane - head);
+    // sorted position inside the tile -> its storage slot
+    const int t = key / kTileCells;
+    const int ts = tile_start[t], nt = tile_start[t + 1] - ts;
+    const int o = ts + (int)((int64_t)(base + (lane - head) - ts) * tile_mult[t] % nt);
     dst.cell[o] = c;
     dst.ox[o] = src.ox[p];
     dst.oy[o] = src.oy[p];
@@ -122,7 +126,33 @@ __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real
     if (src.id) dst.id[o] = src.id[p];
 }
 
-__global__ void k_tile_chunks(const int *__restrict__ offsets, int ntiles, int *tile_start, int *chunk_n)
+// Storage order inside a tile.  The n particles of a tile, sorted by cell,
+// are stored so that slot s holds sorted position (s * a) mod n, with a the
+// first of kSpread coprime to n.  A wavefront reading 64 consecutive slots
+// then gets particles a positions apart in cell order, i.e. from distinct
+// cells while a cell holds fewer than a particles: its LDS atomics in the
+// push hit distinct addresses, and its memory accesses stay contiguous.
+// The scatter applies the inverse map, position e -> slot e * a^-1 mod n.
+constexpr int kSpread[8] = {37, 41, 43, 47, 53, 59, 61, 67};
+
+// a^-1 mod n for gcd(a, n) = 1 (extended Euclid); 0 for n = 1
+__device__ inline int mod_inverse(int a, int n)
+{
+    int64_t t = 0, nt = 1, r = n, nr = a % n;
+    while (nr != 0) {
+        const int64_t q = r / nr;
+        const int64_t t2 = t - q * nt;
+        t = nt;
+        nt = t2;
+        const int64_t r2 = r - q * nr;
+        r = nr;
+        nr = r2;
+    }
+    return (int)(t < 0 ? t + n : t);
+}
+
+__global__ void k_tile_chunks(const int *__restrict__ offsets, int ntiles, int *tile_start, int *chunk_n,
+                              int *tile_mult)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > ntiles) return;
@@ -131,6 +161,13 @@ __global__ void k_tile_chunks(const int *__restrict__ offsets, int ntiles, int *
     if (t < ntiles) {
         const int cnt = offsets[(t + 1) * kTileCells] - s;
         chunk_n[t] = (cnt + kChunk - 1) / kChunk;
+        int m = 1;   // identity if every spread shares a factor with cnt
+        for (int a : kSpread)
+            if (cnt > 1 && cnt % a != 0) {
+                m = mod_inverse(a, cnt);
+                break;
+            }
+        tile_mult[t] = m;
     } else {
         chunk_n[t] = 0;
     }
@@ -201,10 +238,12 @@ hipError_t launch_sort(hipStream_t st, const TileGeom &tg, const Species<real> &
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if ((e = launch_exclusive_scan(st, b.counts, b.offsets, tg.nkeys, b.scratch)) != hipSuccess) return e;
-    k_tile_chunks<<<nblocks((size_t)tg.ntiles + 1, 256), 256, 0, st>>>(b.offsets, tg.ntiles, b.tile_start, b.chunk_n);
+    k_tile_chunks<<<nblocks((size_t)tg.ntiles + 1, 256), 256, 0, st>>>(b.offsets, tg.ntiles, b.tile_start, b.chunk_n,
+                                                                       b.tile_mult);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (n > 0) {
-        k_scatter<real><<<nblocks((size_t)n, 256), 256, 0, st>>>(src, dst, n, tg, b.offsets);
+        k_scatter<real><<<nblocks((size_t)n, 256), 256, 0, st>>>(src, dst, n, tg, b.offsets, b.tile_start,
+                                                                 b.tile_mult);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if ((e = launch_exclusive_scan(st, b.chunk_n, b.chunk_off, tg.ntiles, b.scratch)) != hipSuccess) return e;

@@ -1,0 +1,468 @@
+// qk_setup.cpp -- config variables -> libquillk.so run description.  Each
+// block follows the part of QUILL's init() named in its comment; units are
+// converted exactly as there (lengths to c/omega by 2 pi, "um" and "fs"
+// conversions with the laser wavelength in cm).
+#include "qk_setup.h"
+
+#include <cmath>
+#include <cstdio>
+
+namespace qkd {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kNcrCoef = 1.11485e13;          // n_cr lambda^2, cm^-1 (src/main.cpp:2589)
+constexpr double kLightCm = 2.99792458e10;       // cm/s
+constexpr double kProtonMass = 1836.1526721;     // m_p / m_e (include/physical_constants.h:4)
+constexpr double kComptonCm = 2.4263086e-10;     // lambda_C, cm (src/main.cpp:2276)
+
+struct Ctx {
+    const std::vector<ConfVar> &v;
+    double lambda = 0;   // cm
+    const ConfVar &get(const char *name, size_t from = 0) const { return find(v, name, from); }
+};
+
+// a length in lambda units (optionally um or fs) to c/omega
+double length(const Ctx &c, const ConfVar &var, bool allow_fs)
+{
+    double x = var.value;
+    if (var.units == "um") x = x * 1e-4 / c.lambda;
+    else if (allow_fs && var.units == "fs") x = x * 1e-15 * kLightCm / c.lambda;
+    return x * 2 * kPi;
+}
+
+// convert_units(value, units, "1/k") (src/main.cpp:2465-2491), with "#"
+// (a plain number) read as `dflt` as get_units does (:2493-2495)
+bool time_to_k(const Ctx &c, const ConfVar &var, const char *dflt, double &out, std::string &err)
+{
+    const std::string u = var.units == "#" ? std::string(dflt) : var.units;
+    double x = var.value;
+    if (u == "um") x *= 2 * kPi * 1e-4 / c.lambda;
+    else if (u == "mm") x *= 2 * kPi * 0.1 / c.lambda;
+    else if (u == "cm") x *= 2 * kPi / c.lambda;
+    else if (u == "lambda") x *= 2 * kPi;
+    else {
+        err = "unknown time units [" + u + "] for " + var.name;
+        return false;
+    }
+    out = x;
+    return true;
+}
+
+bool find_boolean(const Ctx &c, const char *name, bool dflt)
+{
+    const std::string &u = c.get(name).units;
+    if (u == "on" || u == "true") return true;
+    if (u == "off" || u == "false") return false;
+    return dflt;
+}
+
+// the last iteration of OutputIterations (src/parameters.cpp:6-38)
+int last_iteration(const std::vector<double> &t_end, const std::vector<double> &period, double dt)
+{
+    int previous = -1;
+    double t_begin = 0;
+    for (size_t n = 0; n < t_end.size(); ++n) {
+        const int count = (int)((t_end[n] + 0.5 * dt - t_begin) / period[n]) + 1;
+        for (int j = 0; j < count; ++j) {
+            const int it = (int)std::round((t_begin + j * period[n]) / dt);
+            if (it > previous) previous = it;
+        }
+        t_begin = t_end[n];
+    }
+    const int final_it = (int)std::round(t_end.back() / dt);
+    return final_it > previous ? final_it : previous;
+}
+
+std::vector<std::string> split_amp(const std::string &s)
+{
+    std::vector<std::string> out(3);
+    size_t n = 0, b = 0;
+    for (size_t i = 0; i <= s.size() && n < 3; ++i)
+        if (i == s.size() || s[i] == '&') {
+            out[n++] = s.substr(b, i - b);
+            b = i + 1;
+        }
+    return out;
+}
+
+}  // namespace
+
+bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
+{
+    Ctx c{vars};
+    qk_params &p = run.params;
+    p = qk_params{};
+
+    // grid steps and wavelength (src/main.cpp:2573-2614)
+    double dx = c.get("dx").value * 2 * kPi, dy = c.get("dy").value * 2 * kPi, dz = c.get("dz").value * 2 * kPi;
+    double dt = c.get("dt").value * 2 * kPi;
+    const ConfVar &lam = c.get("lambda");
+    c.lambda = lam.units == "um" ? lam.value * 1e-4 : lam.value;
+    const ConfVar &nev = c.get("ne");
+    double ne = nev.units == "ncr" ? nev.value * kNcrCoef / c.lambda / c.lambda : nev.value;
+    if (lam.units == "lambda_p") c.lambda = lam.value * std::sqrt(kNcrCoef / ne);
+    if (c.lambda <= 0) {
+        err = "lambda must be positive";
+        return false;
+    }
+    const double k0 = std::sqrt(kNcrCoef / c.lambda / c.lambda / ne);
+    if (dt == 0) {
+        dt = 0.5 * k0 * std::sqrt(4.04);
+        dx = 0;
+    }
+    if (dx == 0) {
+        dx = (1 + 1e-4) * dt / (1 - 1 / (k0 * k0) * dt * dt / 4.04);
+        if (dx < 0) {
+            err = "cannot calculate stable dx, reduce dt";
+            return false;
+        }
+    }
+    run.lambda_cm = c.lambda;
+
+    // box (src/main.cpp:2616-2644)
+    const double xlength = length(c, c.get("xlength"), true);
+    const double ylength = length(c, c.get("ylength"), false);
+    const double zlength = length(c, c.get("zlength"), false);
+
+    // run length (src/main.cpp:2646-2663)
+    {
+        std::vector<double> t_end, period;
+        size_t from = 0;
+        do {
+            const size_t it = find_index(vars, "t_end", from);
+            if (it == vars.size()) {
+                err = "t_end is not set";
+                return false;
+            }
+            double te, op;
+            if (!time_to_k(c, vars[it], "lambda", te, err)) return false;
+            from = it + 1;
+            const ConfVar &opv = c.get("output_period", from);
+            if (opv.units == "t_end") op = opv.value * te;
+            else if (!time_to_k(c, opv, "lambda", op, err)) return false;
+            if (!(op > 0)) {
+                err = "output_period must be positive";
+                return false;
+            }
+            t_end.push_back(te);
+            period.push_back(op);
+        } while (c.get("t_end", from).value != 0);
+        run.last_iteration = last_iteration(t_end, period, dt);
+    }
+
+    // laser (src/main.cpp:2681-2905)
+    const double xsigma = length(c, c.get("xsigma"), true);
+    const double ysigma = length(c, c.get("ysigma"), false);
+    const double zsigma = length(c, c.get("zsigma"), false);
+    double x0 = length(c, c.get("x0"), true);
+    double a0y = 0, a0z = 0;
+    {
+        const ConfVar &P = c.get("P");
+        double pw = P.value * (P.units == "PW" ? 1e22 : (P.units == "TW" ? 1e19 : 1));
+        if (pw != 0) a0y = 5.27281e+17 * std::sqrt(2 * pw / (kPi * 2.42161e+52 * ysigma * zsigma));
+        const ConfVar &I = c.get("I");
+        double iv = I.value * (I.units == "W/cm^2" ? 1e7 : 1);
+        if (iv != 0) a0y = 5.27281e+17 * c.lambda * std::sqrt(iv / (kPi * 2.42161e+52));
+        const ConfVar &W = c.get("W");
+        double wv = W.value * (W.units == "J" ? 1e7 : 1);
+        if (wv != 0) {
+            a0y = std::sqrt(wv / (3.691e4 * c.lambda * kPi * std::sqrt(kPi / 2) * xsigma * ysigma * zsigma / 4));
+            if (c.get("polarization").units == "circular") {
+                a0y = a0y / std::sqrt(2);
+                a0z = a0y;
+            }
+        }
+        if (c.get("a0").value != 0) a0y = c.get("a0").value;
+        const ConfVar &pol = c.get("polarization");
+        if (pol.units == "circular") a0z = a0y;
+        if (pol.units == "elliptic") a0z = pol.value * a0y;
+        if (c.get("a0y").value != 0) {
+            a0y = c.get("a0y").value;
+            a0z = c.get("a0z").value;
+        } else if (c.get("a0z").value != 0) {
+            a0z = c.get("a0z").value;
+            a0y = 0;
+        }
+    }
+    p.external_bz = c.get("external_bz").value;
+    std::string f_envelope = c.get("f_envelope").units;
+    if (f_envelope.empty()) f_envelope = "cos";
+    if (f_envelope != "cos") {
+        err = "f_envelope = " + f_envelope + " is not supported (only the cos envelope with sscos = 0)";
+        return false;
+    }
+    const double b_sign = c.get("b_sign").value != -1 ? 1 : 0;
+    auto angle = [&](const char *name) {
+        const ConfVar &v = c.get(name);
+        return v.units == "pi" ? v.value * kPi : v.value;
+    };
+    const double phase = angle("phase"), phi = angle("phi"), phi_rotate = angle("phi_rotate");
+    double y00 = length(c, c.get("y0"), false), z00 = length(c, c.get("z0"), false);
+    {
+        const ConfVar &r0v = c.get("r0");
+        if (r0v.units != "off" && r0v.value != 0) {
+            const double r0 = length(c, r0v, false);
+            const ConfVar &th = c.get("theta");
+            const double theta = th.units == "deg" ? th.value * kPi / 180 : th.value;
+            x0 = -r0 * std::cos(theta) + 0.00001;
+            y00 = -r0 * std::sin(theta);
+        }
+    }
+    double target[3] = {0, 0, 0};
+    {
+        const char *names[3] = {"xtarget", "ytarget", "ztarget"};
+        for (int a = 0; a < 3; ++a) {
+            const ConfVar &t = c.get(names[a]);
+            if (t.units != "off" && t.value != 0) target[a] = length(c, t, false);
+        }
+    }
+
+    // options outside this library's per-step path are refused
+    {
+        const ConfVar &mw = c.get("mwindow");
+        const bool flows = c.get("nelflow").value != 0 || c.get("nerflow").value != 0;
+        if (mw.units != "off" && !flows) {
+            err = "the moving window is not supported; set mwindow = off";
+            return false;
+        }
+        if (flows) {
+            err = "neutral flows (nelflow / nerflow) are not supported";
+            return false;
+        }
+        if (c.get("beam").units == "on") {
+            err = "beam = on is not supported";
+            return false;
+        }
+        const std::string &pm = c.get("pmerging").units;
+        if (!pm.empty() && pm != "off") {
+            err = "particle merging (pmerging = " + pm + ") is not supported";
+            return false;
+        }
+        if (c.get("catching").units == "on" || c.get("dump_photons").units == "on") {
+            err = "catching / dump_photons are not supported";
+            return false;
+        }
+        if (c.get("shenergy").value != 0) {
+            err = "second harmonic (shenergy) is not supported";
+            return false;
+        }
+    }
+
+    // films (src/main.cpp:3053-3231) and ion populations (:3251-3330)
+    const std::string ions = c.get("ions").units;
+    {
+        std::vector<Film> films;   // in config order
+        size_t from = 0;
+        for (;;) {
+            size_t at = from;
+            for (;;) {
+                at = find_index(vars, "film", at);
+                if (at == vars.size() || vars[at].units == "on") break;
+                ++at;
+            }
+            if (at == vars.size()) break;
+            from = at + 1;
+            Film f{};
+            auto len = [&](const char *name, bool fs) {
+                const ConfVar &v = c.get(name, from);
+                if (v.units == "dx" && std::string(name) == "x0film") return v.value * dx;
+                return length(c, v, fs);
+            };
+            f.x0 = len("x0film", true);
+            f.x1 = f.x0 + len("filmwidth", true);
+            f.gradwidth = len("gradwidth", true);
+            f.gradwidth_y = len("gradwidth_y", true);
+            f.y0 = len("y0film", true);
+            f.y1 = c.get("y1film", from).value == 0 ? ylength : len("y1film", true);
+            f.z0 = len("z0film", true);
+            f.z1 = c.get("z1film", from).value == 0 ? zlength : len("z1film", true);
+            auto dens = [&](const char *name) {
+                const ConfVar &v = c.get(name, from);
+                if (v.units == "ncr") return v.value * kNcrCoef / c.lambda / c.lambda;
+                if (v.units == "ne") return v.value * ne;
+                return v.value;
+            };
+            double nf = dens("nfilm"), n0 = dens("nfilm_y0"), n1 = dens("nfilm_y1");
+            if (nf != 0) n0 = n1 = nf;
+            const double to_ncr = 1.0 / (kNcrCoef / c.lambda / c.lambda);
+            f.ne_y0 = n0 * to_ncr;
+            f.ne_y1 = n1 * to_ncr;
+            const double mcr = c.get("mcr", from).value == 0 ? 1 : c.get("mcr", from).value;
+            f.cmr = 1 / (kProtonMass * mcr);
+            f.ions = ions == "on";
+            f.T = c.get("Tfilm", from).value;
+            f.vx = c.get("vxfilm", from).value;
+            f.npic[0] = (int)c.get("xnpic_film", from).value;
+            f.npic[1] = (int)c.get("ynpic_film", from).value;
+            f.npic[2] = (int)c.get("znpic_film", from).value;
+            if (f.T != 0) {
+                err = "thermal films (Tfilm != 0) are not supported";
+                return false;
+            }
+            films.push_back(f);
+            run.ion_mcr.push_back(mcr);
+        }
+        // the reference keeps films newest first (p_last_film, prev links)
+        run.films.assign(films.rbegin(), films.rend());
+        std::vector<double> mcrs;
+        for (auto it = run.ion_mcr.rbegin(); it != run.ion_mcr.rend(); ++it) {
+            bool seen = false;
+            for (double m : mcrs) seen = seen || m == *it;
+            if (!seen) mcrs.push_back(*it);
+        }
+        run.ion_mcr = (ions == "on" || ions == "positrons") ? mcrs : std::vector<double>{};
+    }
+    const int xnpic = (int)c.get("xnpic").value, ynpic = (int)c.get("ynpic").value, znpic = (int)c.get("znpic").value;
+    for (Film &f : run.films)
+        if (f.npic[0] == 0 || f.npic[1] == 0 || f.npic[2] == 0) {
+            f.npic[0] = xnpic;
+            f.npic[1] = ynpic;
+            f.npic[2] = znpic;
+        }
+    if ((int)run.ion_mcr.size() > QK_MAX_ION) {
+        err = "more than " + std::to_string(QK_MAX_ION) + " ion populations";
+        return false;
+    }
+    run.data_folder = c.get("data_folder").units.empty() ? "results" : c.get("data_folder").units;
+
+    // physics options (src/main.cpp:3531-3598)
+    p.qed = find_boolean(c, "qed", true) ? 1 : 0;
+    std::string pusher = c.get("pusher").units, solver = c.get("solver").units;
+    if (pusher.empty()) pusher = "vay";
+    if (solver.empty()) solver = "ndfx";
+    if (pusher == "boris") p.pusher = QK_PUSHER_BORIS;
+    else if (pusher == "vay") p.pusher = QK_PUSHER_VAY;
+    else {
+        err = "pusher unknown: " + pusher;
+        return false;
+    }
+    if (solver == "ndfx") p.solver = QK_SOLVER_NDFX;
+    else if (solver == "fdtd") p.solver = QK_SOLVER_FDTD;
+    else if (solver == "five_point") p.solver = QK_SOLVER_FIVE_POINT;
+    else {
+        err = "solver unknown: " + solver;
+        return false;
+    }
+
+    // subdomain (src/main.cpp:2241-2243, n_sr = 1) and the per-step constants
+    // of the loop body (:2323-2327)
+    p.nx = (int)(xlength / dx);
+    p.ny = (int)(ylength / dy);
+    p.nz = (int)(zlength / dz);
+    p.dx = dx;
+    p.dy = dy;
+    p.dz = dz;
+    p.dt = dt;
+    p.e_s = c.lambda / kComptonCm;
+    p.q_vac = 8 * kPi * kPi / (dx * dy * dz) * 2.818e-13 / c.lambda;
+    p.norm_n = dx * dy * dz * kNcrCoef * c.lambda / (8 * kPi * kPi * kPi);
+    p.norm_ef = 0.5 * dx * dy * dz * 3.691e4 * c.lambda / 1e7;
+    p.norm_ep = 8.2e-14 * dx * dy * dz * kNcrCoef * c.lambda / (8 * kPi * kPi * kPi);
+    p.n_ion = (int)run.ion_mcr.size();
+    for (int n = 0; n < p.n_ion; ++n) p.ion_cmr[n] = 1 / (kProtonMass * run.ion_mcr[n]);
+    p.precision = QK_FP32;
+    p.deposit = QK_DEPOSIT_TILED;
+    p.sort_every = 20;
+
+    // laser calls of init_fields, cos envelope (src/main.cpp:1150-1227)
+    auto call = [&](double ay, double az, double xc, double xs, double y, double z, double append, double ph) {
+        LaserCall l{{ay, az, xsigma, ysigma, zsigma, xc, b_sign, xs, phase, y, z, append, ph, target[0], target[1],
+                     target[2]}};
+        run.lasers.push_back(l);
+    };
+    if (a0y != 0 || a0z != 0) {
+        if (phi_rotate != 0) {
+            call(a0y, a0z, xlength / 2 + x0, x0, y00, z00, 1, phi_rotate);
+        } else {
+            call(a0y, a0z, xlength / 2 + x0, x0, y00, z00, 1, 0);
+            std::string lp = c.get("lp_reflection").units, fr = c.get("f_reflection").units;
+            if (lp.empty()) lp = "off";
+            if (fr.empty()) fr = "off";
+            const std::vector<std::string> L = split_amp(lp), F = split_amp(fr);
+            // mirrored copy: amplitude signs from the y / z flags, +x0 or -x0
+            auto mirror = [&](const std::string &fy, const std::string &fz, int sx, int sy, int sz) {
+                call((1 - 2 * (fy == "y")) * a0y, (1 - 2 * (fz == "z")) * a0z, xlength / 2 + sx * x0, sx * x0,
+                     sy * y00, sz * z00, 1, 0);
+            };
+            if (phi != 0) {
+                call(a0y, a0z, xlength / 2 - x0, -x0, -y00, -z00, 1, phi);
+            } else if (L[0] == "xy") {
+                mirror(F[0], F[0], +1, +1, -1);
+                if (L[1] == "xz") {
+                    mirror(F[1], F[1], +1, -1, +1);
+                    mirror(F[1], F[1], +1, -1, -1);
+                    if (L[2] == "yz") {
+                        mirror(F[2], F[1], -1, +1, +1);
+                        mirror(F[2], F[1], -1, +1, -1);
+                        mirror(F[2], F[1], -1, -1, +1);
+                        mirror(F[2], F[1], -1, -1, -1);
+                    }
+                } else if (L[1] == "yz") {
+                    mirror(F[1], F[1], -1, +1, +1);
+                    mirror(F[1], F[1], -1, +1, -1);
+                    if (L[2] == "xz") {
+                        mirror(F[2], F[1], -1, -1, +1);
+                        mirror(F[2], F[1], +1, -1, +1);
+                        mirror(F[2], F[1], -1, -1, -1);
+                        mirror(F[2], F[1], +1, -1, -1);
+                    }
+                }
+            } else if (L[0] == "xz") {
+                mirror(F[0], F[0], +1, -1, +1);
+                if (L[1] == "yz") {
+                    mirror(F[1], F[1], -1, -1, +1);
+                    mirror(F[1], F[1], -1, +1, +1);
+                }
+            } else if (L[0] == "yz") {
+                mirror(F[0], F[0], -1, +1, +1);
+            }
+        }
+    }
+    return true;
+}
+
+void film_particles(const Run &run, const Film &f, double cmr, ParticleSet &out)
+{
+    // film() (src/init.cpp:537-646) with T = 0 and no transverse profile
+    const qk_params &p = run.params;
+    const double dx = p.dx, dy = p.dy, dz = p.dz;
+    int i0 = (int)(f.x0 / dx), i1 = (int)(f.x1 / dx);
+    i0 = std::min(std::max(i0, 0), p.nx);
+    i1 = std::min(std::max(i1, 0), p.nx);
+    const int jy0 = (int)(f.y0 / dy), jy1 = (int)(f.y1 / dy);
+    const int kz0 = (int)(f.z0 / dz), kz1 = (int)(f.z1 / dz);
+    const double film_ylen = double(jy1 - jy0 - 3);
+    const double ux0 = f.vx / std::sqrt(1 - f.vx * f.vx);
+    const int bi = f.npic[0], bj = f.npic[1], bk = f.npic[2];
+    const bool electron = cmr < 0;
+    for (int i = i0; i < i1; ++i)
+        for (int j = jy0 + 1; j < jy1 - 1; ++j) {
+            double nes = f.ne_y1 * (j - jy0 - 1) / film_ylen + f.ne_y0 * (jy1 - 2 - j) / film_ylen;
+            for (int k = kz0 + 1; k < kz1 - 1; ++k) {
+                // the y ramps scale nes once per k, cumulatively, as the reference does
+                if (f.gradwidth_y >= 0 && j - jy0 - 1 < f.gradwidth_y / dy) nes *= (j - jy0 - 1) / (f.gradwidth_y / dy);
+                if (f.gradwidth_y >= 0 && jy1 - 2 - j < f.gradwidth_y / dy) nes *= (jy1 - 2 - j) / (f.gradwidth_y / dy);
+                double n;
+                if (f.gradwidth >= 0) n = i >= (int)((f.x0 + f.gradwidth) / dx) ? nes : nes * (i * dx - f.x0) / f.gradwidth;
+                else n = i >= (int)((f.x0 - f.gradwidth) / dx) ? nes : nes * (1 + (i * dx - f.x0) / f.gradwidth);
+                // fill_cell_by_particles (src/init.cpp:402-478), dsplmt = 0
+                double q0 = 1 / double(bi * bj * bk) * n;
+                if (electron) q0 = -q0;
+                for (int ip = 0; ip < bi; ++ip)
+                    for (int jp = 0; jp < bj; ++jp)
+                        for (int kp = 0; kp < bk; ++kp) {
+                            out.x.push_back(0.5 / bi + double(i) + double(ip) / double(bi));
+                            out.y.push_back(0.5 / bj + double(j) + double(jp) / double(bj));
+                            out.z.push_back(0.5 / bk + double(k) + double(kp) / double(bk));
+                            out.ux.push_back(ux0);
+                            out.uy.push_back(0);
+                            out.uz.push_back(0);
+                            out.q.push_back(q0);
+                        }
+            }
+        }
+}
+
+}  // namespace qkd

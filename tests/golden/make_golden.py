@@ -5,7 +5,7 @@ built by oracle/Makefile from the reference tree) on small seeded cases and
 stores inputs plus the reference's outputs as .npz files next to this script.
 Run in the build container only (the reference tree is not on the GPU host):
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [case ...]
 
 Cases
   run_*      full steps (src/main.cpp:2319-2332 order) from random fields and a
@@ -15,6 +15,9 @@ Cases
   laser_cos  f_init_cos (src/init.cpp:6-210), sscos = 0, tilted and rotated
   rates      chi / w / tilde_w / mathcal_W (src/mc.cpp:26-116) on a table
   push       push_boris / push_vay (src/pusher.cpp) on a table
+  qed_stats  QED on (photon emission, Breit-Wheeler): per-step diagnostics and
+             the photon-gamma histogram of 16 runs with distinct seeds of the
+             reference's own random stream (statistical parity)
 """
 import os
 import subprocess
@@ -29,6 +32,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
 
 import golden_io as gio  # noqa: E402
+import qed_case  # noqa: E402
 import oracle  # noqa: E402
 
 PI = np.pi
@@ -157,20 +161,50 @@ def case_push():
     print("push: done")
 
 
-def main():
+def case_qed_stats():
+    """QED statistics of the reference over qed_case.SEEDS seeds of its own
+    random stream: per-step diagnostics and the run summaries."""
+    c = qed_case.inputs()
+    diags, summ = [], []
+    for seed in range(1, qed_case.SEEDS + 1):
+        h = gio.header(mode=gio.MODE_RUN, nx=c["nx"], ny=c["ny"], nz=c["nz"], solver=oracle.FDTD,
+                       pusher=oracle.VAY, qed=1, nsteps=c["nsteps"], n_particles=len(c["rec"]), seed=seed, n_ion=0,
+                       dx=c["dx"], dy=c["dx"], dz=c["dx"], dt=c["dt"], e_s=qed_case.E_S,
+                       **norms(c["dx"], c["dx"], c["dx"]))
+        _, prec, _, diag = run_ref(h, c["E"], c["B"], c["rec"], c["ids"])
+        ph = prec[prec[:, 8] == 0]
+        # diag row: N_e N_p N_ph n_e n_p n_ph energy_f energy_e energy_p energy_ph ...
+        diags.append(diag)
+        summ.append(qed_case.summary(diag[-1][5], diag[-1][4], diag[-1][9], ph[:, 6]))
+    np.savez_compressed(os.path.join(HERE, "qed_stats.npz"), header=h.tobytes(), diag=np.array(diags),
+                        summary=np.array(summ))
+    s = np.array(summ)
+    print(f"qed_stats: {qed_case.SEEDS} seeds, final photons {s[:, 0].mean():.0f}, positrons {s[:, 1].mean():.1f}")
+
+
+def main(names):
+    """Generate the named fixtures (all when none are named)."""
     if not os.path.exists(oracle.REF_BIN):
         sys.exit("oracle/_ref/quill_ref missing: run `make -C oracle` with the reference tree present")
     dx = 0.05 * 2 * PI
     dt = 0.025 * 2 * PI
-    case_run("run_fdtd_boris", 14, 12, 10, dx, dx, dx, dt, oracle.FDTD, oracle.BORIS, 10, 101, 0.8, 0.5)
-    case_run("run_ndfx_vay", 12, 14, 10, dx, 1.2 * dx, 1.4 * dx, 0.9 * dt, oracle.NDFX, oracle.VAY, 10, 202,
-             0.8, 0.5, external_bz=0.3)
-    case_run("run_five_vay", 12, 10, 12, dx, dx, dx, dt, oracle.FIVE_POINT, oracle.VAY, 10, 303, 0.8, 0.5)
-    case_run("run_fdtd_vay_cold", 12, 12, 12, dx, dx, dx, dt, oracle.FDTD, oracle.VAY, 12, 404, 0.01, 0.05)
-    case_laser()
-    case_rates()
-    case_push()
+    cases = {
+        "run_fdtd_boris": lambda: case_run("run_fdtd_boris", 14, 12, 10, dx, dx, dx, dt, oracle.FDTD, oracle.BORIS,
+                                           10, 101, 0.8, 0.5),
+        "run_ndfx_vay": lambda: case_run("run_ndfx_vay", 12, 14, 10, dx, 1.2 * dx, 1.4 * dx, 0.9 * dt, oracle.NDFX,
+                                         oracle.VAY, 10, 202, 0.8, 0.5, external_bz=0.3),
+        "run_five_vay": lambda: case_run("run_five_vay", 12, 10, 12, dx, dx, dx, dt, oracle.FIVE_POINT, oracle.VAY,
+                                         10, 303, 0.8, 0.5),
+        "run_fdtd_vay_cold": lambda: case_run("run_fdtd_vay_cold", 12, 12, 12, dx, dx, dx, dt, oracle.FDTD,
+                                              oracle.VAY, 12, 404, 0.01, 0.05),
+        "laser_cos": case_laser,
+        "rates": case_rates,
+        "push": case_push,
+        "qed_stats": case_qed_stats,
+    }
+    for name in names or list(cases):
+        cases[name]()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -1,0 +1,165 @@
+"""quillk-run, the .conf-driven driver: config grammar and the derived run.
+
+The grammar and storage follow parse.py:8-66 and var::read
+(src/low-level.cpp:376-444); the derived quantities follow init()
+(src/main.cpp:2558-3600).  Known answers are worked out by hand from those
+rules (values pass through float as stof does).  No GPU is touched
+(--print-conf / --dry-run).
+"""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "quill_amd", "bin", "quillk-run")
+TWO_PI = 2 * math.pi
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def run(tmp_path, text, *args, ok=True):
+    conf = tmp_path / "run.conf"
+    conf.write_text(text)
+    r = subprocess.run([BIN, str(conf), *args], capture_output=True, text=True, timeout=60)
+    if ok:
+        assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    if not os.path.exists(BIN):
+        pytest.skip("quillk-run not built (make -C quill_amd/driver)")
+
+
+def test_grammar(tmp_path):
+    text = ("# comment line\n"
+            "a = 1.5   # trailing comment\n"
+            "b = [1;2.5;-3e2]\n"
+            "c = on\n"
+            "d = 0.91 um\n"
+            "e = 4e20 W/cm^2\n"
+            "f = 1e\n"
+            "g = 5 cm^{-3}\n"
+            "h = 12 %\n"
+            "i = .5\n"
+            "j = xy&xz\n")
+    v = json.loads(run(tmp_path, text, "--print-conf").stdout)
+    got = {x["name"]: (x["value"], x["units"], x["array"]) for x in v}
+    assert got["a"] == (1.5, "#", [])                     # plain number: units "#"
+    assert got["b"] == (0.0, "#", [1.0, 2.5, -300.0])     # arrays in double
+    assert got["c"] == (0.0, "on", [])                    # text: value 0, text in units
+    assert got["d"] == (f32(0.91), "um", [])              # number through float (stof)
+    assert got["e"] == (f32(4e20), "W/cm^2", [])
+    assert got["f"] == (0.0, "1e", [])                    # not a number: text
+    assert got["g"] == (5.0, "cm^{-3}", [])
+    assert got["h"] == (12.0, "%", [])
+    assert got["i"] == (0.5, "#", [])
+    assert got["j"] == (0.0, "xy&xz", [])
+    assert [x["name"] for x in v] == list("abcdefghij")   # order kept (find takes the first)
+
+
+@pytest.mark.parametrize("line, msg", [("a = eval{1+1}", "eval"), ("a = 1 = 2", "key = value"),
+                                       ("a = 1 um m", "incorrect"), ("a b = 1", "bad key")])
+def test_grammar_errors(tmp_path, line, msg):
+    r = run(tmp_path, line + "\n", "--print-conf", ok=False)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+BASE = """\
+dx = 0.05
+dy = 0.1
+dz = 0.1
+dt = 0.025
+lambda = 1 um
+ne = 0.01 ncr
+xlength = 3.2001
+ylength = 2.4001
+zlength = 2.4001
+t_end = 0.5
+output_period = 0.25
+f_envelope = cos
+polarization = linear
+a0 = 1
+xsigma = 1
+ysigma = 1
+zsigma = 1
+x0 = -0.5
+mwindow = off
+qed = off
+solver = fdtd
+pusher = boris
+film = on
+x0film = 1
+filmwidth = 1
+gradwidth = 0
+nfilm = 0.01 ncr
+xnpic = 2
+ynpic = 1
+znpic = 1
+"""
+
+
+def test_derived_run(tmp_path):
+    j = json.loads(run(tmp_path, BASE, "--dry-run").stdout)
+    dx, dy, dt = f32(0.05) * TWO_PI, f32(0.1) * TWO_PI, f32(0.025) * TWO_PI
+    assert j["dx"] == pytest.approx(dx, rel=1e-15) and j["dy"] == pytest.approx(dy, rel=1e-15)
+    assert j["dt"] == pytest.approx(dt, rel=1e-15)
+    assert (j["nx"], j["ny"], j["nz"]) == (64, 24, 24)
+    # OutputIterations: last iteration round(t_end / dt) = round(0.5 / 0.025) = 20 -> 21 steps
+    assert j["steps"] == 21
+    assert (j["solver"], j["pusher"], j["qed"]) == ("fdtd", "boris", 0)
+    lam = f32(1) * 1e-4
+    assert j["lambda_cm"] == pytest.approx(lam, rel=1e-15)
+    assert j["e_s"] == pytest.approx(lam / 2.4263086e-10, rel=1e-15)
+    vol = dx * dy * dy
+    assert j["norm_n"] == pytest.approx(vol * 1.11485e13 * lam / (8 * math.pi ** 3), rel=1e-14)
+    assert j["norm_ef"] == pytest.approx(0.5 * vol * 3.691e4 * lam / 1e7, rel=1e-14)
+    # one laser call: f_init_cos(a0, 0, xs, ys, zs, xlength/2 + x0, b_sign=1, x0, phase 0, 0, 0, append, phi 0)
+    assert len(j["lasers"]) == 1
+    L = j["lasers"][0]
+    xl = f32(3.2001) * TWO_PI
+    x0 = f32(-0.5) * TWO_PI
+    np.testing.assert_allclose(L[:8], [1.0, 0.0, TWO_PI, TWO_PI, TWO_PI, xl / 2 + x0, 1.0, x0], rtol=1e-6)
+    # film (src/init.cpp:537-646): cells i in [int(x0/dx), int(x1/dx)), j in
+    # [int(y0/dy) + 1, int(y1/dy) - 1) with y0 = 0, y1 = ylength, same for k;
+    # 2 particles each, q = -n / 2 per electron (place() later drops the
+    # cells outside is_inside; the set-up counts all)
+    i0 = int(f32(1) * TWO_PI / dx)
+    i1 = int((f32(1) * TWO_PI + f32(1) * TWO_PI) / dx)
+    nj = int(f32(2.4001) * TWO_PI / dy) - 2
+    cells = (i1 - i0) * nj * nj
+    assert j["particles"]["0"]["n"] == 2 * cells
+    assert j["particles"]["0"]["q_sum"] == pytest.approx(-f32(0.01) * cells, rel=1e-12)
+
+
+def test_reflections_and_circular(tmp_path):
+    text = BASE.replace("polarization = linear", "polarization = circular") + "lp_reflection = xy&yz\nf_reflection = y\n"
+    j = json.loads(run(tmp_path, text, "--dry-run").stdout)
+    # base pulse + xy mirror + two yz mirrors (src/main.cpp:1183-1195)
+    assert len(j["lasers"]) == 4
+    a = np.array(j["lasers"])
+    assert a[0, 0] == 1.0 and a[0, 1] == 1.0          # circular: a0z = a0y
+    assert a[1, 0] == -1.0                            # f_reflection = y flips a0y of the first mirror
+    assert a[2, 7] == pytest.approx(-a[0, 7]) and a[3, 7] == pytest.approx(-a[0, 7])   # yz mirrors at -x0
+    assert a[1, 10] == pytest.approx(-a[0, 10])       # xy mirror flips z0
+
+
+@pytest.mark.parametrize("extra, msg", [("mwindow = on\n", "moving window"), ("beam = on\n", "beam"),
+                                        ("pmerging = ti\n", "merging"), ("f_envelope = focused\n", "focused"),
+                                        ("solver = spectral\n", "solver unknown")])
+def test_unsupported_options_are_refused(tmp_path, extra, msg):
+    # find() takes the first occurrence, so the override goes first
+    r = run(tmp_path, extra + BASE, "--dry-run", ok=False)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_qed_defaults_on(tmp_path):
+    j = json.loads(run(tmp_path, BASE.replace("qed = off\n", ""), "--dry-run").stdout)
+    assert j["qed"] == 1   # find_boolean(first, "qed", true), src/main.cpp:3542

@@ -1,0 +1,80 @@
+"""QED on the GPU: photon emission, Breit-Wheeler pairs and vacuum pairs.
+
+Event sequences cannot match the reference (its get_rand is a sequential
+lagged-Fibonacci stream, src/mc.cpp:12-23; the GPU uses counter-based Philox
+streams), so parity is statistical: 16 seeded GPU runs against the 16
+reference runs of tests/golden/qed_stats.npz (made by the reference's own
+code, tests/golden/make_golden.py), |z| < 4 on every summary.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+import golden_io as gio  # noqa: E402
+import oracle  # noqa: E402
+import qed_case  # noqa: E402
+import quill_amd as qa  # noqa: E402
+
+
+def gpu_summaries(precision):
+    c = qed_case.inputs()
+    h = np.frombuffer(gio.load("qed_stats")["header"].tobytes(), gio.HDR)[0]
+    out = []
+    for seed in range(1, qed_case.SEEDS + 1):
+        with qa.Region(c["nx"], c["ny"], c["nz"], c["dx"], c["dx"], c["dx"], c["dt"], solver="fdtd", pusher="vay",
+                       precision=precision, qed=True, e_s=qed_case.E_S, seed=seed, norm_n=float(h["norm_n"]),
+                       norm_ef=float(h["norm_ef"]), norm_ep=float(h["norm_ep"])) as r:
+            r.set_fields(c["E"], c["B"])
+            r.add_records(c["rec"], c["ids"])
+            r.init_boundaries()
+            d = r.step(c["nsteps"], diag=True)[-1]
+            ph, _ = r.particles(qa.PHOTON)
+            assert d["n_ph"] == len(ph)
+            out.append(qed_case.summary(d["n_ph"], d["n_p"], d["energy_ph"], ph[:, 6]))
+    return np.array(out)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_qed_statistics_match_reference(precision):
+    print(f"QED statistics, {precision}, {qed_case.SEEDS} seeds", flush=True)
+    ref = gio.load("qed_stats")["summary"]
+    mine = gpu_summaries(precision)
+    z = qed_case.zscores(mine, ref)
+    bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES, z) if not abs(v) < 4]
+    assert not bad, f"GPU QED statistics differ from the reference: {bad}"
+
+
+@pytest.mark.gpu
+def test_vacuum_pair_rate():
+    """birth_from_vacuum (src/mc.cpp:142-196): in a uniform E the number of
+    pairs born in one step is binomial over the cells 2 .. n-3 with the
+    per-cell probability W dt dV of the Schwinger rate (oracle rate, pinned
+    to the reference by rates.npz)."""
+    n = 24
+    dx = 0.05 * 2 * np.pi
+    dt = 0.025 * 2 * np.pi
+    e_s, e0 = 1.0, 10.0
+    E = np.zeros((n, n, n, 3))
+    E[..., 0] = e0
+    B = np.zeros_like(E)
+    w = oracle.lib().oq_vacuum_rate(e_s, oracle._dp(np.array([e0, 0.0, 0.0])), oracle._dp(np.zeros(3)))
+    p = w * dt * dx ** 3
+    cells = (n - 4) ** 3
+    counts = []
+    for seed in range(1, 9):
+        with qa.Region(n, n, n, dx, dx, dx, dt, solver="fdtd", pusher="boris", precision="fp32", qed=True, e_s=e_s,
+                       seed=seed, q_vac=1e-3) as r:
+            r.set_fields(E, B)
+            r.step(1)
+            counts.append((r.count(qa.ELECTRON), r.count(qa.POSITRON)))
+    counts = np.array(counts, float)
+    assert (counts[:, 0] == counts[:, 1]).all()   # pairs
+    mean, sd = cells * p, np.sqrt(cells * p * (1 - p))
+    z = (counts[:, 1].mean() - mean) / (sd / np.sqrt(len(counts)))
+    assert mean > 20 and abs(z) < 4, (counts[:, 1], mean, z)

@@ -113,3 +113,36 @@ def test_pushers_match_reference():
         uv, gv = oracle.push(oracle.VAY, cmr, dt, e, b, u, g)
         np.testing.assert_allclose(np.r_[ub, gb], out[:4], rtol=1e-12, atol=1e-14)
         np.testing.assert_allclose(np.r_[uv, gv], out[4:], rtol=1e-12, atol=1e-14)
+
+
+def qed_oracle_summaries():
+    import qed_case
+    c = qed_case.inputs()
+    h = hdr(gio.load("qed_stats"))
+    out = []
+    for seed in range(1, qed_case.SEEDS + 1):
+        o = oracle.Oracle(c["nx"], c["ny"], c["nz"], c["dx"], c["dx"], c["dx"], c["dt"], solver=oracle.FDTD,
+                          pusher=oracle.VAY, qed=True, e_s=qed_case.E_S, seed=seed, norm_n=float(h["norm_n"]),
+                          norm_ef=float(h["norm_ef"]), norm_ep=float(h["norm_ep"]))
+        o.field(oracle.E)[:] = c["E"]
+        o.field(oracle.B)[:] = c["B"]
+        o.add_particles(c["rec"], c["ids"])
+        o.init_boundaries()
+        d = o.step(c["nsteps"], diag=True)[-1]
+        rec, _ = o.particles()
+        out.append(qed_case.summary(d.n_ph, d.n_p, d.energy_ph, rec[rec[:, 8] == 0][:, 6]))
+    return np.array(out)
+
+
+def test_oracle_qed_statistics_match_reference():
+    """Photon emission and pair creation of the oracle (its own random stream)
+    against 16 runs of the reference (get_rand, src/mc.cpp:12-23): final
+    photon and positron counts, photon energy and the photon-gamma histogram
+    agree as distributions (|z| < 4 on each of 11 summaries)."""
+    import qed_case
+    ref = gio.load("qed_stats")["summary"]
+    mine = qed_oracle_summaries()
+    z = qed_case.zscores(mine, ref)
+    assert ref[:, 0].mean() > 10000 and ref[:, 1].mean() > 10   # the case exercises both processes
+    bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES, z) if not abs(v) < 4]
+    assert not bad, f"oracle QED statistics differ from the reference: {bad}"
