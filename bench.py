@@ -186,6 +186,27 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "rc": rc}), flush=True)
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of k_push_tiled from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
+    scaled by the factors a plain copy of known size reads on the same
+    counters, both stated in the summary), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"k_push_tiled_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            for key in ("FETCH_SIZE_KiB_per_dispatch", "WRITE_SIZE_KiB_per_dispatch", "FETCH_SIZE_factor",
+                        "WRITE_SIZE_factor"):
+                if line.strip().startswith(key + ":"):
+                    vals[key] = float(line.split(":", 1)[1].split()[0])
+    if len(vals) < 4:
+        return None
+    return 1024.0 * (vals["FETCH_SIZE_KiB_per_dispatch"] * vals["FETCH_SIZE_factor"] +
+                     vals["WRITE_SIZE_KiB_per_dispatch"] * vals["WRITE_SIZE_factor"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -236,7 +257,7 @@ def main():
                    "particles": main_run["n_start"], "solver": w["solver"], "pusher": w["pusher"], "qed": False,
                    "deposit": "tiled", "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_push (gather+push+deposit)", "avg_launch_ms": push_ms,
                      "algorithmic_bytes_per_launch": push_bytes},
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,

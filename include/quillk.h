@@ -77,6 +77,8 @@ typedef struct {
     double norm_n;                /* compute_N multiplier, src/main.cpp:2326 (0 -> 1) */
     double norm_ef;               /* compute_energy field multiplier, src/main.cpp:2327 (0 -> 1) */
     double norm_ep;               /* compute_energy particle multiplier (0 -> 1) */
+    int32_t merging;              /* pmerging after a step that crosses the limit: 0 off, 1 "ti", 2 "nl" */
+    double crnp;                  /* the limit: crpc x cells (src/main.cpp:1701), macro-particles per species */
 } qk_params;
 
 /* per-step diagnostics, compute_N / compute_energy (src/fout.cpp:41-138) */
@@ -135,6 +137,35 @@ int qk_get_particles(qk_ctx *ctx, int species, int64_t cap, double *rec10, int32
 /* compute_rho (src/fout.cpp:7-39): node charge of electrons, positrons and
  * photons in the three slots of rho (AoS [nx][ny][nz][3]) */
 int qk_compute_rho(qk_ctx *ctx, double *rho);
+
+/* spatial_region::pmerging (src/mc.cpp:202-462): thin every species once;
+ * mode 1 "ti" deletes each particle with probability ppd[0], mode 2 "nl" a
+ * particle of species s with probability ppd[s] / (1 + (2|q|/<|q|>_s)^4),
+ * and both rescale each species' charges to keep its total.  ppd has
+ * QK_ION0 + n_ion entries (electrons, positrons, photons, ions).  qk_step
+ * calls it by itself when params.merging is set (the trigger of
+ * evaluate_merging_condition, src/main.cpp:1691-1748). */
+int qk_pmerging(qk_ctx *ctx, int mode, const double *ppd);
+
+/* spatial_region::moving_window (src/advance.cpp:389-454): erase the
+ * particles of the plane i = 0, move E, B, Be and every particle one cell
+ * toward -x (x -= 1), zero the plane nx - 1.  J is left as it is.  The caller
+ * decides when (check_moving_window, src/main.cpp:1953-1978) and refills the
+ * front (add_moving_window_particles, :1750-1835), as main() does. */
+int qk_moving_window(qk_ctx *ctx);
+/* spatial_region::get_max_w (src/low-level.cpp:512-532): the largest
+ * E^2 + B^2 over the planes ilo <= i < ihi (the "auto" window trigger) */
+int qk_max_w(qk_ctx *ctx, int ilo, int ihi, double *max_w);
+/* fill_cell_by_particles (src/init.cpp:402-478) with T = 0 on a list of cells:
+ * cells holds ncells (i, j, k) triples, density their n; each cell gets
+ * npic[0] x npic[1] x npic[2] particles at 0.5/b + cell + index/b per axis,
+ * u = (ux0, 0, 0), charge density / (npic product) (negative for cmr < 0).
+ * Cells outside is_inside are skipped (place() deletes their particles).
+ * With track_ids, the particles of the m-th listed cell get the ids
+ * first_id + m x npt + (ip npic[1] + jp) npic[2] + kp.  *added (may be NULL)
+ * receives the number stored. */
+int qk_fill_cells(qk_ctx *ctx, int species, int64_t ncells, const int32_t *cells, const double *density,
+                  const int32_t npic[3], double ux0, int32_t first_id, int64_t *added);
 
 /* timing of the last qk_step call, measured with HIP events on the context's
  * stream: total milliseconds and the part spent in the particle kernels */

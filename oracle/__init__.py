@@ -75,6 +75,12 @@ def lib():
         L.oq_rate_w.argtypes = [C.c_double, C.c_double, dp, C.c_double]
         L.oq_rate_tilde_w.restype = C.c_double
         L.oq_rate_tilde_w.argtypes = [C.c_double] * 4
+        L.oq_pmerging.argtypes = [vp, C.c_int, dp, C.c_int, dp]
+        L.oq_moving_window.argtypes = [vp]
+        L.oq_max_w.restype = C.c_double
+        L.oq_max_w.argtypes = [vp, C.c_int, C.c_int]
+        L.oq_fill_cells.restype = C.c_int64
+        L.oq_fill_cells.argtypes = [vp, C.c_double, C.c_int64, ip, dp, ip, C.c_double, C.c_int32]
         L.oq_vacuum_rate.restype = C.c_double
         L.oq_vacuum_rate.argtypes = [C.c_double, dp, dp]
         _lib = L
@@ -161,6 +167,25 @@ class Oracle:
                   append=True, phi=0.0, xtarget=0.0, ytarget=0.0, ztarget=0.0):
         lib().oq_init_laser_cos(self.h, a0y, a0z, xsigma, ysigma, zsigma, x0, int(b_sign), x1, phase, y0, z0,
                                 int(append), phi, xtarget, ytarget, ztarget)
+
+    def pmerging(self, mode, ppd, ion_cmr=()):
+        """One merge (mode "ti" or "nl") with deletion fractions ppd[3 + n_ion]."""
+        ppd = np.ascontiguousarray(ppd, np.float64)
+        icmr = np.ascontiguousarray(list(ion_cmr) + [0.0], np.float64)
+        lib().oq_pmerging(self.h, 1 if mode == "ti" else 2, _dp(ppd), len(ion_cmr), _dp(icmr))
+
+    def moving_window(self):
+        lib().oq_moving_window(self.h)
+
+    def max_w(self, ilo, ihi):
+        return lib().oq_max_w(self.h, ilo, ihi)
+
+    def fill_cells(self, cmr, cells, density, npic, ux0=0.0, first_id=0):
+        """fill_cell_by_particles on listed cells (int (m, 3)), density (m,)."""
+        cells = np.ascontiguousarray(cells, np.int32)
+        density = np.ascontiguousarray(density, np.float64)
+        npic = np.ascontiguousarray(npic, np.int32)
+        return lib().oq_fill_cells(self.h, cmr, len(cells), _ip(cells), _dp(density), _ip(npic), ux0, first_id)
 
     def load_uniform(self, ppc, density, sigma_u, seed, ilo=1, ihi=1 << 30):
         return lib().oq_load_uniform(self.h, ppc, density, sigma_u, seed, ilo, ihi)

@@ -25,7 +25,12 @@ enum {
     QG_MODE_LASER = 1,    /* f_init_cos only, dump E and B */
     QG_MODE_RATES = 2,    /* chi / w / tilde_w / mathcal_W on the rows of a table */
     QG_MODE_PUSH = 3,     /* pushers on the rows of a table */
-    QG_MODE_TIME = 4      /* timing: synthetic plasma + laser, warm-up + timed steps */
+    QG_MODE_TIME = 4,     /* timing: synthetic plasma + laser, warm-up + timed steps */
+    QG_MODE_FOUT = 5,     /* output writers: fields + particles in, the reference's plane files out */
+    QG_MODE_MERGE = 6,    /* pmerging once: particles in (ppd in laser[], ti/nl in extra[0]), particles out */
+    QG_MODE_MW = 7,       /* RUN with a moving window; extra: mode (1 on, 2 auto), mwspeed, mwtolerance,
+                             column density, npic x y z, t_add_mw; per step nmw and the auto ratio follow the diagnostics */
+    QG_MODE_FILM = 8      /* one film() call: arguments in laser[], append flag and gradwidth_y in extra[0..1] */
 };
 
 typedef struct {
@@ -42,7 +47,7 @@ typedef struct {
     double norm_n, norm_ef, norm_ep;
     double laser[16];        /* a0y a0z xsigma ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt */
     double ion_cmr[4];
-    double extra[8];         /* TIME: ppc, density, sigma_u, warmup, ilo, ihi */
+    double extra[8];         /* TIME: ppc, density, sigma_u, warmup, ilo, ihi; FOUT: i_yz, binary */
 } qg_header;
 
 /* diagnostics row: N_e N_p N_ph n_e n_p n_ph energy_f energy_e energy_p energy_ph energy_ion n_ion */

@@ -157,6 +157,73 @@ static void compact(oq_state *s)
     s->np = m;
 }
 
+/* moving_window (advance.cpp:389-454): the particles of the plane i = 0
+ * are erased, E, B and Be move one plane toward -x (F[i] = F[i + 1]), the
+ * plane nx - 1 is zeroed and every particle's x drops by one.  J stays. */
+void oq_moving_window(oq_state *s)
+{
+    const int nx = s->p.nx;
+    const int64_t plane = (int64_t)s->p.ny * s->p.nz * 3;
+    for (int64_t n = 0; n < s->np; ++n)
+        if (s->part[n].alive && (int)s->part[n].x == 0) s->part[n].alive = 0;
+    compact(s);
+    const int moved[3] = {OQ_E, OQ_B, OQ_BE};
+    for (int a = 0; a < 3; ++a) {
+        double *F = s->f[moved[a]];
+        memmove(F, F + plane, (size_t)((int64_t)(nx - 1) * plane) * sizeof(double));
+        memset(F + (int64_t)(nx - 1) * plane, 0, (size_t)plane * sizeof(double));
+    }
+    for (int64_t n = 0; n < s->np; ++n) s->part[n].x -= 1;
+}
+
+/* get_max_w (low-level.cpp:516-532) */
+double oq_max_w(const oq_state *s, int ilo, int ihi)
+{
+    double m = 0;
+    for (int i = ilo; i < ihi; ++i)
+        for (int j = 0; j < s->p.ny; ++j)
+            for (int k = 0; k < s->p.nz; ++k) {
+                const double *e = &s->f[OQ_E][IDX(s, i, j, k)], *b = &s->f[OQ_B][IDX(s, i, j, k)];
+                double w = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+                if (w > m) m = w;
+            }
+    return m;
+}
+
+/* fill_cell_by_particles (init.cpp:402-478) with T = 0 and dsplmt = 0 on
+ * listed cells; cell m's particles get ids first_id + m * npt + their index
+ * in (ip, jp, kp) order; place() drops the cells outside is_inside */
+int64_t oq_fill_cells(oq_state *s, double cmr, int64_t ncells, const int32_t *cells, const double *density,
+                      const int npic[3], double ux0, int32_t first_id)
+{
+    const int bi = npic[0], bj = npic[1], bk = npic[2], npt = bi * bj * bk;
+    int64_t kept = 0;
+    for (int64_t m = 0; m < ncells; ++m) {
+        const int ci = cells[3 * m], cj = cells[3 * m + 1], ck = cells[3 * m + 2];
+        if (!is_inside(s, ci, cj, ck)) continue;
+        double q0 = 1 / (double)npt * density[m];
+        if (cmr < 0) q0 = -q0;
+        for (int ip = 0; ip < bi; ++ip)
+            for (int jp = 0; jp < bj; ++jp)
+                for (int kp = 0; kp < bk; ++kp) {
+                    oq_particle *p = new_particle(s);
+                    p->cmr = cmr;
+                    p->q = q0;
+                    p->x = (0.5 / bi + (double)ci) + (double)ip / (double)bi;
+                    p->y = (0.5 / bj + (double)cj) + (double)jp / (double)bj;
+                    p->z = (0.5 / bk + (double)ck) + (double)kp / (double)bk;
+                    p->ux = ux0;
+                    p->uy = 0;
+                    p->uz = 0;
+                    p->g = cmr != 0 ? sqrt(1 + ux0 * ux0) : sqrt(ux0 * ux0);
+                    p->chi = 0;
+                    p->id = first_id + (int32_t)(m * npt) + (ip * bj + jp) * bk + kp;
+                    ++kept;
+                }
+    }
+    return kept;
+}
+
 /* ------------------------------------------------------------------ */
 /* pushers (pusher.cpp)                                               */
 
@@ -831,6 +898,73 @@ void oq_padvance(oq_state *s)
             jdeposition(s, p, d);
             place(s, p);
         }
+    }
+    compact(s);
+}
+
+/* pmerging (mc.cpp:202-462) */
+static int merge_species(double cmr, int n_ion, const double *ion_cmr)
+{
+    if (cmr == -1) return 0;
+    if (cmr == 1) return 1;
+    if (cmr == 0) return 2;
+    for (int n = 0; n < n_ion; ++n)
+        if (cmr == ion_cmr[n]) return 3 + n;
+    return -1;
+}
+
+/* _ppd (mc.cpp:198-200) */
+static double merge_weight(double q, double q0)
+{
+    return 1 / (1 + pow(2 * q / q0, 4));
+}
+
+void oq_pmerging(oq_state *s, int mode, const double *ppd, int n_ion, const double *ion_cmr)
+{
+    enum { NS = 3 + 4 };
+    double tot[NS] = {0}, del[NS] = {0}, cnt[NS] = {0};
+    const int ns = 3 + n_ion;
+    for (int64_t n = 0; n < s->np; ++n) {
+        oq_particle *p = &s->part[n];
+        const int sp = p->alive ? merge_species(p->cmr, n_ion, ion_cmr) : -1;
+        if (sp < 0) continue;
+        tot[sp] += fabs(p->q);
+        cnt[sp] += 1;
+    }
+    if (mode == 1) {
+        for (int64_t n = 0; n < s->np; ++n) {
+            oq_particle *p = &s->part[n];
+            const int sp = p->alive ? merge_species(p->cmr, n_ion, ion_cmr) : -1;
+            if (sp < 0) continue;
+            if (oq_rand(s) < ppd[0]) {
+                del[sp] += fabs(p->q);
+                p->alive = 0;
+            }
+        }
+        for (int sp = 0; sp < ns; ++sp)
+            if (tot[sp] - del[sp] != 0) tot[sp] = tot[sp] / (tot[sp] - del[sp]);
+    } else {
+        double av[NS], kept[NS] = {0};
+        for (int sp = 0; sp < ns; ++sp) av[sp] = tot[sp] / cnt[sp];
+        for (int64_t n = 0; n < s->np; ++n) {
+            oq_particle *p = &s->part[n];
+            const int sp = p->alive ? merge_species(p->cmr, n_ion, ion_cmr) : -1;
+            if (sp < 0) continue;
+            if (oq_rand(s) < ppd[sp] * merge_weight(fabs(p->q), av[sp])) p->alive = 0;
+        }
+        for (int64_t n = 0; n < s->np; ++n) {
+            oq_particle *p = &s->part[n];
+            const int sp = p->alive ? merge_species(p->cmr, n_ion, ion_cmr) : -1;
+            if (sp < 0) continue;
+            p->q = p->q / (1 - ppd[sp] * merge_weight(fabs(p->q), av[sp]));
+            kept[sp] += fabs(p->q);
+        }
+        for (int sp = 0; sp < ns; ++sp) tot[sp] = kept[sp] != 0 ? tot[sp] / kept[sp] : 0;
+    }
+    for (int64_t n = 0; n < s->np; ++n) {
+        oq_particle *p = &s->part[n];
+        const int sp = p->alive ? merge_species(p->cmr, n_ion, ion_cmr) : -1;
+        if (sp >= 0) p->q *= tot[sp];
     }
     compact(s);
 }

@@ -67,6 +67,24 @@ void oq_interpolate_be(oq_state *s);      /* advance.cpp:20-83 */
 void oq_maxwell(oq_state *s);             /* maxwell.cpp:10-22 */
 void oq_padvance(oq_state *s);            /* advance.cpp:90-387 */
 void oq_birth_from_vacuum(oq_state *s);   /* mc.cpp:142-196 */
+/* pmerging (mc.cpp:202-462): mode 1 "ti" deletes every particle with
+ * probability ppd[0]; mode 2 "nl" deletes a particle of species s with
+ * probability ppd[s] / (1 + (2 |q| / <|q|>_s)^4) and divides the survivors'
+ * charge by one minus that probability; both then rescale each species so
+ * that its total charge is unchanged.  Species: 0 electrons, 1 positrons,
+ * 2 photons, 3 + n ions of charge-to-mass ratio ion_cmr[n]. */
+void oq_pmerging(oq_state *s, int mode, const double *ppd, int n_ion, const double *ion_cmr);
+/* moving_window (advance.cpp:389-454): erase plane i = 0, move E, B, Be and
+ * the particles one cell toward -x, zero plane nx - 1 */
+void oq_moving_window(oq_state *s);
+/* get_max_w (low-level.cpp:516-532): max of E^2 + B^2 over planes [ilo, ihi) */
+double oq_max_w(const oq_state *s, int ilo, int ihi);
+/* fill_cell_by_particles (init.cpp:402-478) with T = 0 on ncells cells (i j k
+ * triples): npic[0] x npic[1] x npic[2] particles each at 0.5 / b + cell +
+ * index / b, u = (ux0, 0, 0), q = +-density / (npic product) (negative for
+ * cmr < 0), ids first_id + m * npt + particle index.  Returns particles kept. */
+int64_t oq_fill_cells(oq_state *s, double cmr, int64_t ncells, const int32_t *cells, const double *density,
+                      const int npic[3], double ux0, int32_t first_id);
 void oq_diagnostics(oq_state *s, oq_diag *d);   /* fout.cpp:41-138 */
 void oq_compute_rho(oq_state *s);         /* fout.cpp:7-39 (rho stored in J slots) */
 /* one full step in the order of src/main.cpp:2319-2332; diag may be NULL */

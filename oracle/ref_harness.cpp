@@ -15,6 +15,8 @@
  * Case-file layout: oracle/golden_format.h.
  */
 #include <chrono>
+#include <fstream>
+#include <string>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -292,6 +294,170 @@ int main(int argc, char **argv)
                     (long long)np, (long long)h.nx * h.ny * h.nz);
         double res[2] = {sec, (double)np};
         wr(fo, res, sizeof res);
+        delete sr;
+    } else if (h.mode == QG_MODE_FOUT) {
+        /* the plane files of write_fields / write_density (src/main.cpp:369-476,
+         * 745-995) for n_sr = 1, written by the reference's own spatial_region
+         * writers next to the output file (<out>.ex ...); the output file gets
+         * E, B, rho (cj after compute_rho) and Be */
+        std::vector<double> icmr;
+        spatial_region *sr = make_region(h, icmr);
+        std::vector<double> E(nf), B(nf);
+        rd(fi, E.data(), nf * 8);
+        rd(fi, B.data(), nf * 8);
+        put_fields(sr, h, E.data(), B.data());
+        std::vector<double> pr((size_t)h.n_particles * 8);
+        std::vector<int32_t> ids((size_t)h.n_particles);
+        rd(fi, pr.data(), pr.size() * 8);
+        rd(fi, ids.data(), ids.size() * 4);
+        for (int n = 0; n < h.n_particles; ++n) add_particle(sr, &pr[(size_t)n * 8], ids[n]);
+        set_start(sr, h);   /* compute_rho walks the lists from pl.start (src/fout.cpp:29) */
+        sr->interpolate_be();
+        const int iyz = (int)h.extra[0];
+        const std::ios_base::openmode mode =
+            h.extra[1] != 0 ? (std::ios_base::out | std::ios_base::binary) : std::ios_base::out;
+        const std::string base = argv[2];
+        typedef void (spatial_region::*plane_fn)(std::ofstream *, int, int, std::ios_base::openmode);
+        typedef void (spatial_region::*yz_fn)(std::ofstream *, int, std::ios_base::openmode);
+        const char *names[6] = {"ex", "ey", "ez", "bx", "by", "bz"};
+        const plane_fn planes[6] = {&spatial_region::fout_ex, &spatial_region::fout_ey, &spatial_region::fout_ez,
+                                    &spatial_region::fout_bx, &spatial_region::fout_by, &spatial_region::fout_bz};
+        const yz_fn yz[6] = {&spatial_region::fout_ex_yzplane, &spatial_region::fout_ey_yzplane,
+                             &spatial_region::fout_ez_yzplane, &spatial_region::fout_bx_yzplane,
+                             &spatial_region::fout_by_yzplane, &spatial_region::fout_bz_yzplane};
+        for (int c = 0; c < 6; ++c) {
+            const std::string name = base + "." + names[c];
+            {
+                std::ofstream f(name.c_str(), mode);
+                (sr->*planes[c])(&f, 0, h.nx, mode);
+            }
+            std::ofstream f(name.c_str(), mode | std::ios_base::app);
+            (sr->*yz[c])(&f, iyz, mode | std::ios_base::app);
+        }
+        for (int inv = 0; inv < 2; ++inv) {
+            const std::string name = base + (inv ? ".inv" : ".w");
+            {
+                std::ofstream f(name.c_str(), std::ios_base::out);
+                if (inv) sr->fout_inv(&f, 0, h.nx, true);
+                else sr->fout_w(&f, 0, h.nx, true);
+            }
+            std::ofstream f(name.c_str(), std::ios_base::out | std::ios_base::app);
+            if (inv) sr->fout_inv_yzplane(&f, iyz);
+            else sr->fout_w_yzplane(&f, iyz);
+        }
+        sr->compute_rho();
+        {
+            std::ofstream fx((base + ".rho").c_str(), mode), fy((base + ".rho_p").c_str(), mode),
+                fz((base + ".rho_ph").c_str(), mode);
+            sr->fout_rho(&fx, &fy, &fz, 0, h.nx, mode);
+        }
+        {
+            const std::ios_base::openmode am = mode | std::ios_base::app;
+            std::ofstream fx((base + ".rho").c_str(), am), fy((base + ".rho_p").c_str(), am),
+                fz((base + ".rho_ph").c_str(), am);
+            sr->fout_rho_yzplane(&fx, &fy, &fz, iyz, am);
+        }
+        dump_fields(sr, h, fo, true);
+        delete sr;
+    } else if (h.mode == QG_MODE_MERGE) {
+        /* one call of pmerging (src/mc.cpp:202-462) on the given particles with
+         * ppd = laser[0 .. 3 + n_ion) and the "ti" (extra[0] = 1) or "nl" (2)
+         * algorithm, then the surviving particles */
+        std::vector<double> icmr;
+        spatial_region *sr = make_region(h, icmr);
+        std::vector<double> pr((size_t)h.n_particles * 8);
+        std::vector<int32_t> ids((size_t)h.n_particles);
+        rd(fi, pr.data(), pr.size() * 8);
+        rd(fi, ids.data(), ids.size() * 4);
+        for (int n = 0; n < h.n_particles; ++n) add_particle(sr, &pr[(size_t)n * 8], ids[n]);
+        set_start(sr, h);
+        std::vector<double> ppd(h.laser, h.laser + 3 + (h.n_ion > 0 ? h.n_ion : 0));
+        sr->pmerging(ppd.data(), h.extra[0] == 1 ? "ti" : "nl");
+        dump_particles(sr, h, fo);
+        delete sr;
+    } else if (h.mode == QG_MODE_MW) {
+        /* RUN with a moving window, in the order of main()'s loop
+         * (src/main.cpp:2333-2336, 2396-2398): after each step
+         * check_moving_window (:1953-1978, "on" with mwspeed or "auto" with
+         * mwtolerance), moving_window(), then while (l + 1) dt < t_add_mw a
+         * column of fill_cell_by_particles at i = nx - 3 (:1750-1786, uniform
+         * density, no profiles).  New particles get ids base + f ny nz npt +
+         * (j nz + k) npt + creation index for the f-th fill.  After the
+         * diagnostics: per step nmw and the auto-mode ratio. */
+        std::vector<double> icmr;
+        spatial_region *sr = make_region(h, icmr);
+        std::vector<double> E(nf), B(nf);
+        rd(fi, E.data(), nf * 8);
+        rd(fi, B.data(), nf * 8);
+        put_fields(sr, h, E.data(), B.data());
+        std::vector<double> pr((size_t)h.n_particles * 8);
+        std::vector<int32_t> ids((size_t)h.n_particles);
+        rd(fi, pr.data(), pr.size() * 8);
+        rd(fi, ids.data(), ids.size() * 4);
+        for (int n = 0; n < h.n_particles; ++n) add_particle(sr, &pr[(size_t)n * 8], ids[n]);
+        set_start(sr, h);
+        sr->f_init_boundaries();
+        sr->interpolate_be();
+        const int mw = (int)h.extra[0];
+        const double mwspeed = h.extra[1], tol = h.extra[2], dens = h.extra[3], t_add = h.extra[7];
+        int_vector3d npic;
+        npic.i = (int)h.extra[4];
+        npic.j = (int)h.extra[5];
+        npic.k = (int)h.extra[6];
+        const int npt = npic.i * npic.j * npic.k;
+        const int id_base = 1 << 24;
+        int nmw = 1, fills = 0;
+        std::vector<double> diag((size_t)h.nsteps * QG_DIAG_WIDTH), shifts(2 * (size_t)h.nsteps);
+        for (int l = 0; l < h.nsteps; ++l) {
+            one_step(sr, h, &diag[(size_t)l * QG_DIAG_WIDTH]);
+            bool shift = false;
+            double ratio = 0;
+            if (mw == 1) {
+                shift = (l + 1) * h.dt * mwspeed > nmw * h.dx;
+            } else if (mw == 2) {
+                double max_w = sr->get_max_w();
+                ratio = max_w > 0 ? sr->get_max_w(h.nx - nm - 2, h.nx - nm) / max_w : 0.0;
+                shift = ratio > tol;
+            }
+            if (shift) {
+                sr->moving_window();
+                ++nmw;
+            }
+            shifts[2 * (size_t)l] = nmw;
+            shifts[2 * (size_t)l + 1] = ratio;
+            if (shift && (l + 1) * h.dt < t_add && dens > 0) {
+                int_vector3d a;
+                a.i = h.nx - 3;
+                for (int j = 0; j < h.ny; ++j)
+                    for (int k = 0; k < h.nz; ++k) {
+                        a.j = j;
+                        a.k = k;
+                        sr->fill_cell_by_particles(-1, a, npic, dens);
+                        if (!sr->is_inside(a.i, j, k)) continue;
+                        /* place() put each new particle at the head of the list */
+                        particle *p = sr->cp[a.i][j][k].pl.head;
+                        for (int e = 0; e < npt && p; ++e, p = p->next)
+                            p->trn = id_base + (fills * h.ny * h.nz + j * h.nz + k) * npt + (npt - 1 - e);
+                    }
+                ++fills;
+            }
+        }
+        dump_fields(sr, h, fo, true);
+        dump_particles(sr, h, fo);
+        wr(fo, diag.data(), diag.size() * 8);
+        wr(fo, shifts.data(), shifts.size() * 8);
+        delete sr;
+    } else if (h.mode == QG_MODE_FILM) {
+        /* one call of film() (src/init.cpp:537-646) on an empty region with
+         * the arguments laser[0..15] = x0 x1 ne_y0 ne_y1 ions cmr gradwidth y0
+         * y1 z0 z1 T vx xnpic ynpic znpic, extra[0] = append_for_moving_window,
+         * extra[1] = gradwidth_y (is_profiled off), then the particles */
+        std::vector<double> icmr;
+        spatial_region *sr = make_region(h, icmr);
+        const double *L = h.laser;
+        sr->film(L[0], L[1], L[2], L[3], (int)L[4], L[5], L[6], L[7], L[8], L[9], L[10], L[11], L[12], false,
+                 (int)L[13], (int)L[14], (int)L[15], h.extra[0] != 0, h.extra[1]);
+        dump_particles(sr, h, fo);
         delete sr;
     } else {
         die("unknown mode");

@@ -32,7 +32,7 @@ SPECIES_NAMES = {"electron": ELECTRON, "positron": POSITRON, "photon": PHOTON}
 EXPORTS = ["qk_create", "qk_destroy", "qk_last_error", "qk_set_fields", "qk_get_fields", "qk_add_particles",
            "qk_load_uniform_plasma", "qk_init_laser_cos", "qk_init_boundaries", "qk_step", "qk_sync",
            "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_last_fallback",
-           "qk_get_stream"]
+           "qk_get_stream", "qk_pmerging", "qk_moving_window", "qk_max_w", "qk_fill_cells"]
 
 
 class QuillError(RuntimeError):
@@ -49,7 +49,8 @@ class Params(C.Structure):
                 ("e_s", C.c_double), ("external_bz", C.c_double), ("q_vac", C.c_double),
                 ("seed", C.c_uint64), ("capacity", C.c_int64),
                 ("n_ion", C.c_int32), ("ion_cmr", C.c_double * MAX_ION),
-                ("norm_n", C.c_double), ("norm_ef", C.c_double), ("norm_ep", C.c_double)]
+                ("norm_n", C.c_double), ("norm_ef", C.c_double), ("norm_ep", C.c_double),
+                ("merging", C.c_int32), ("crnp", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -96,6 +97,10 @@ def load_library(path=LIB_PATH):
     L.qk_last_timing.argtypes = [vp, dp, dp, dp]
     L.qk_last_fallback.argtypes = [vp, dp]
     L.qk_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.qk_pmerging.argtypes = [vp, C.c_int, dp]
+    L.qk_moving_window.argtypes = [vp]
+    L.qk_max_w.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.qk_fill_cells.argtypes = [vp, C.c_int, C.c_int64, ip, dp, ip, C.c_double, C.c_int32, i64p]
     for name in EXPORTS:
         if name not in ("qk_last_error",):
             getattr(L, name).restype = C.c_int
@@ -128,7 +133,7 @@ class Region:
     def __init__(self, nx, ny, nz, dx, dy, dz, dt, solver="fdtd", pusher="boris", precision="fp32",
                  deposit="tiled", qed=False, sort_every=0, track_ids=False, device=0, e_s=1e10,
                  external_bz=0.0, q_vac=0.0, seed=1, capacity=0, ion_cmr=(), norm_n=1.0, norm_ef=1.0,
-                 norm_ep=1.0):
+                 norm_ep=1.0, merging=None, crnp=0.0):
         L = load_library()
         p = Params()
         p.nx, p.ny, p.nz = nx, ny, nz
@@ -148,6 +153,8 @@ class Region:
         for n, v in enumerate(ion_cmr):
             p.ion_cmr[n] = v
         p.norm_n, p.norm_ef, p.norm_ep = norm_n, norm_ef, norm_ep
+        p.merging = {None: 0, "off": 0, "ti": 1, "nl": 2}[merging]
+        p.crnp = crnp
         self.params = p
         self.shape = (nx, ny, nz)
         self.precision = precision
@@ -255,6 +262,34 @@ class Region:
         rho = np.zeros(self.shape + (3,))
         _check(load_library().qk_compute_rho(self._h, _dp(rho)), "qk_compute_rho")
         return rho
+
+    def pmerging(self, mode, ppd):
+        """spatial_region::pmerging: mode "ti" or "nl", deletion fractions
+        ppd for electrons, positrons, photons and each ion population."""
+        a = np.zeros(MAX_SPECIES)
+        a[:len(ppd)] = ppd
+        _check(load_library().qk_pmerging(self._h, 1 if mode == "ti" else 2, _dp(a)), "qk_pmerging")
+
+    # -- moving window ------------------------------------------------------
+    def moving_window(self):
+        """spatial_region::moving_window: shift fields and particles one cell toward -x."""
+        _check(load_library().qk_moving_window(self._h), "qk_moving_window")
+
+    def max_w(self, ilo, ihi):
+        """spatial_region::get_max_w over the planes [ilo, ihi)."""
+        v = C.c_double()
+        _check(load_library().qk_max_w(self._h, ilo, ihi, C.byref(v)), "qk_max_w")
+        return v.value
+
+    def fill_cells(self, species, cells, density, npic, ux0=0.0, first_id=0):
+        """fill_cell_by_particles (T = 0) on listed cells (int (m, 3)) with densities (m,)."""
+        cells = np.ascontiguousarray(cells, np.int32)
+        density = np.ascontiguousarray(density, np.float64)
+        npic = np.ascontiguousarray(npic, np.int32)
+        added = C.c_int64()
+        _check(load_library().qk_fill_cells(self._h, species, len(cells), _ip(cells), _dp(density), _ip(npic),
+                                            ux0, first_id, C.byref(added)), "qk_fill_cells")
+        return added.value
 
     # -- stepping -----------------------------------------------------------
     def step(self, nsteps=1, diag=False):

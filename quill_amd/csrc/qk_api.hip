@@ -60,6 +60,7 @@ struct SpeciesBuf {
     int max_chunks = 0;     // capacity of chunks
     int nchunks = 0;        // host copy after the last sort
     int steps_since_sort = 0;
+    int ishift = 0;         // moving-window shifts since the last sort
 };
 
 }  // namespace
@@ -90,6 +91,17 @@ struct qk_ctx {
     int *d_count = nullptr;
     int *h_count = nullptr;
     uint32_t step_index = 0;
+    // merging: device accumulators, their pinned mirror, and the deletion
+    // fractions of a merge due at the start of the next step
+    double *d_merge = nullptr;
+    double *h_merge = nullptr;
+    bool merge_due = false;
+    double merge_ppd[QK_MAX_SPECIES] = {};
+    // moving window: get_max_w result (device, pinned mirror) and the staging
+    // buffer of qk_fill_cells' cell lists
+    unsigned long long *d_maxw = nullptr, *h_maxw = nullptr;
+    void *d_fill = nullptr;
+    size_t fill_cap = 0;
 };
 
 namespace {
@@ -269,6 +281,7 @@ int sort_species(qk_ctx *c, int s)
     b.n_sorted = b.n;
     b.nchunks = c->h_sortinfo[1];
     b.steps_since_sort = 0;
+    b.ishift = 0;
     if (b.nchunks > b.max_chunks) return fail(QK_ERR_STATE, "tile work list overflow");
     return QK_OK;
 }
@@ -306,6 +319,73 @@ int ensure_events(qk_ctx *c, size_t n)
         c->ev.push_back(e);
     }
     return QK_OK;
+}
+
+// pmerging (src/mc.cpp:202-462) on every species; see qk_pmerging
+template <typename real>
+int merge_impl(qk_ctx *c, int mode, const double *ppd)
+{
+    if (!c->d_merge) {
+        HIP_TRY(hipMalloc(&c->d_merge, 4 * sizeof(double)));
+        HIP_TRY(hipHostMalloc((void **)&c->h_merge, 4 * sizeof(double)));
+    }
+    auto reduce = [&](auto &&launch, double out[4]) -> int {
+        HIP_TRY(launch_zero(c->stream, c->d_merge, 4));
+        HIP_TRY(launch());
+        HIP_TRY(hipMemcpyAsync(c->h_merge, c->d_merge, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int a = 0; a < 4; ++a) out[a] = c->h_merge[a];
+        return QK_OK;
+    };
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        SpeciesBuf &b = c->sp[s];
+        if (!b.active || b.n == 0) continue;
+        const double pd = mode == 1 ? ppd[0] : ppd[s];
+        if (!(pd > 0)) continue;
+        const Species<real> v = view<real>(b.buf[b.cur]);
+        double tot[4], got[4];
+        QK_TRY(reduce([&] { return launch_charge_sum<real>(c->stream, v, b.n, c->d_merge); }, tot));
+        if (tot[2] == 0) continue;
+        const double av_q = tot[0] / tot[2];
+        QK_TRY(reduce([&] {
+            return launch_merge_delete<real>(c->stream, v, b.n, mode, pd, av_q, c->p.seed, c->step_index,
+                                             0x400u + (uint32_t)s, c->d_merge);
+        }, got));
+        // "ti": Q / (Q - deleted); "nl": Q / (sum of the raised survivors)
+        double a;
+        if (mode == 1) a = tot[0] - got[0] != 0 ? tot[0] / (tot[0] - got[0]) : 1;
+        else a = got[0] != 0 ? tot[0] / got[0] : 0;
+        HIP_TRY(launch_scale_charge<real>(c->stream, v, b.n, a));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+// evaluate_merging_condition (src/main.cpp:1691-1748) on the counts of the
+// step just finished (h: that step's accumulators)
+void merge_condition(qk_ctx *c, const double *h)
+{
+    const double crnp = c->p.crnp;
+    const int nsp = QK_ION0 + c->p.n_ion;
+    double cnt[QK_MAX_SPECIES] = {}, total = 0;
+    for (int s = 0; s < nsp; ++s) {
+        cnt[s] = h[s * kDiagPerSpecies + 2];
+        total += cnt[s];
+    }
+    c->merge_due = false;
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) c->merge_ppd[s] = 0;
+    if (c->p.merging == 1) {
+        if (total > nsp * crnp) {
+            c->merge_due = true;
+            c->merge_ppd[0] = (total - nsp * crnp) / total;
+        }
+    } else if (c->p.merging == 2) {
+        for (int s = 0; s < nsp; ++s)
+            if (cnt[s] > crnp) {
+                c->merge_due = true;
+                c->merge_ppd[s] = (cnt[s] - crnp) / cnt[s];
+            }
+    }
 }
 
 // QED slot counters: host -> device before kernels that append, device ->
@@ -417,19 +497,27 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
     // the QED kernels deposit through global atomics
     const bool tiled = c->p.deposit == QK_DEPOSIT_TILED && !qed;
     for (int l = 0; l < nsteps; ++l) {
-        // cell-tile sort: particles outside the sorted region, or the cadence
+        // cell-tile sort: the cadence, or an unsorted tail (appended particles,
+        // pushed by the global kernel meanwhile) above 1/16 of the sorted part
         // (photons only with QED, where pair creation leaves free slots)
         for (int s = 0; s < QK_MAX_SPECIES; ++s) {
             SpeciesBuf &b = c->sp[s];
             if (!b.active || b.n == 0 || (b.kind == 1 && !qed)) continue;
             const bool due = c->p.sort_every > 0 && b.steps_since_sort >= c->p.sort_every;
-            if ((tiled && b.n_sorted < b.n) || due) QK_TRY(sort_species<real>(c, s));
+            const bool tail = tiled && (b.n_sorted == 0 || (b.n - b.n_sorted) * 16 > b.n_sorted);
+            if (tail || due) QK_TRY(sort_species<real>(c, s));
+        }
+        // a merge decided after the previous step runs first (pmerging before
+        // birth_from_vacuum, src/main.cpp:2321-2322)
+        if (c->merge_due) {
+            QK_TRY(merge_impl<real>(c, c->p.merging, c->merge_ppd));
+            c->merge_due = false;
         }
         const Current<real> jc = current_of<real>(c, c->jcur), jn = current_of<real>(c, c->jcur ^ 1);
         HIP_TRY(launch_zero(c->stream, c->d_diag, kDiagLen));
-        // padvance (src/advance.cpp:90-387); the boundary erase of p_boundary
-        // (src/plist.cpp:73-102) never finds a particle here: place() keeps only
-        // is_inside cells, which exclude the erased layers
+        // padvance (src/advance.cpp:90-387); the push kernels erase the
+        // particles p_boundary (src/plist.cpp:73-102) would: those outside
+        // is_inside, which only a moving-window shift leaves behind
         HIP_TRY(hipEventRecord(c->ev[3 * l], c->stream));
         if (qed) QK_TRY(qed_particle_phase<real>(c, f, jc));
         for (int s = 0; s < QK_MAX_SPECIES && !qed; ++s) {
@@ -448,6 +536,7 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
                 a.nchunks = b.d_nchunks;
                 a.max_chunks = b.nchunks;
                 a.tg = c->tg;
+                a.tg.ishift = b.ishift;
                 HIP_TRY(launch_push_tiled<real>(c->stream, c->g, f, jc, v, a));
                 a.first = b.n_sorted;   // the unsorted tail, if any
                 a.n = b.n - b.n_sorted;
@@ -472,6 +561,10 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         HIP_TRY(hipMemcpyAsync(hd + (size_t)l * kDiagLen, c->d_diag, kDiagLen * sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));
         if (qed) QK_TRY(qed_download_counts(c));
+        if (c->p.merging) {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            merge_condition(c, hd + (size_t)l * kDiagLen);
+        }
         ++c->step_index;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -566,6 +659,8 @@ int qk_create(const qk_params *params, qk_ctx **out)
     if (p.n_ion < 0 || p.n_ion > QK_MAX_ION) return fail(QK_ERR_INVALID, "n_ion out of range");
     if (p.sort_every < 0) return fail(QK_ERR_INVALID, "sort_every must be >= 0");
     if (p.qed && !(p.e_s > 0)) return fail(QK_ERR_INVALID, "QED needs e_s > 0");
+    if (p.merging < 0 || p.merging > 2) return fail(QK_ERR_INVALID, "merging must be 0, 1 (ti) or 2 (nl)");
+    if (p.merging && !(p.crnp > 0)) return fail(QK_ERR_INVALID, "merging needs crnp > 0");
     // the reference's deposit splits a path at no more than one face per axis
     if (p.dt >= p.dx || p.dt >= p.dy || p.dt >= p.dz)
         return fail(QK_ERR_INVALID, "dt must be below dx, dy and dz (one face crossing per axis and step)");
@@ -596,7 +691,8 @@ int qk_create(const qk_params *params, qk_ctx **out)
     c->tg.nty = (p.ny + kTy - 1) / kTy;
     c->tg.ntz = (p.nz + kTz - 1) / kTz;
     c->tg.ntiles = c->tg.ntx * c->tg.nty * c->tg.ntz;
-    c->tg.nkeys = c->tg.ntiles * kTileCells;
+    c->tg.nkeys = c->tg.ntiles;
+    c->tg.ishift = 0;
     c->rsize = p.precision == QK_FP64 ? 8 : 4;
     // solver coefficients (src/maxwell.cpp:24-33, 238-249)
     for (int a = 0; a < 3; ++a) {
@@ -669,6 +765,11 @@ int qk_destroy(qk_ctx *c)
     if (c->h_diag) note(hipHostFree(c->h_diag));
     if (c->d_count) note(hipFree(c->d_count));
     if (c->h_count) note(hipHostFree(c->h_count));
+    if (c->d_merge) note(hipFree(c->d_merge));
+    if (c->h_merge) note(hipHostFree(c->h_merge));
+    if (c->d_maxw) note(hipFree(c->d_maxw));
+    if (c->h_maxw) note(hipHostFree(c->h_maxw));
+    if (c->d_fill) note(hipFree(c->d_fill));
     for (auto e : c->ev) note(hipEventDestroy(e));
     if (c->stream) note(hipStreamDestroy(c->stream));
     delete c;
@@ -894,6 +995,98 @@ int qk_compute_rho(qk_ctx *c, double *rho)
     const hipError_t ef = hipFree(d);
     HIP_TRY(e);
     HIP_TRY(ef);
+    return QK_OK;
+}
+
+int qk_pmerging(qk_ctx *c, int mode, const double *ppd)
+{
+    if (!c || !ppd) return fail(QK_ERR_INVALID, "null argument");
+    if (mode != 1 && mode != 2) return fail(QK_ERR_INVALID, "merging mode must be 1 (ti) or 2 (nl)");
+    for (int s = 0; s < QK_ION0 + c->p.n_ion; ++s)
+        if (!(ppd[s] >= 0 && ppd[s] < 1)) return fail(QK_ERR_INVALID, "ppd must lie in [0, 1)");
+    return c->rsize == 4 ? merge_impl<float>(c, mode, ppd) : merge_impl<double>(c, mode, ppd);
+}
+
+int qk_moving_window(qk_ctx *c)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (c->rsize == 4) HIP_TRY(launch_shift_fields<float>(c->stream, c->g, fields_of<float>(c)));
+    else HIP_TRY(launch_shift_fields<double>(c->stream, c->g, fields_of<double>(c)));
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        SpeciesBuf &b = c->sp[s];
+        if (!b.active) continue;
+        HIP_TRY(launch_shift_particles(c->stream, b.buf[b.cur].cell, b.n));
+        ++b.ishift;   // the tiles of the last sort moved with their particles
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+int qk_max_w(qk_ctx *c, int ilo, int ihi, double *max_w)
+{
+    if (!c || !max_w) return fail(QK_ERR_INVALID, "null argument");
+    if (ilo < 0 || ihi > c->p.nx) return fail(QK_ERR_INVALID, "planes outside [0, nx)");
+    if (!c->d_maxw) {
+        HIP_TRY(hipMalloc(&c->d_maxw, sizeof(unsigned long long)));
+        HIP_TRY(hipHostMalloc((void **)&c->h_maxw, sizeof(unsigned long long)));
+    }
+    HIP_TRY(launch_fill_int(c->stream, (int *)c->d_maxw, 2, 0));
+    if (c->rsize == 4) HIP_TRY(launch_max_w<float>(c->stream, c->g, fields_of<float>(c), ilo, ihi, c->d_maxw));
+    else HIP_TRY(launch_max_w<double>(c->stream, c->g, fields_of<double>(c), ilo, ihi, c->d_maxw));
+    HIP_TRY(hipMemcpyAsync(c->h_maxw, c->d_maxw, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(max_w, c->h_maxw, sizeof(double));
+    return QK_OK;
+}
+
+int qk_fill_cells(qk_ctx *c, int s, int64_t ncells, const int32_t *cells, const double *density,
+                  const int32_t npic[3], double ux0, int32_t first_id, int64_t *added)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (!species_valid(c, s)) return fail(QK_ERR_INVALID, "invalid species");
+    if (ncells < 0 || (ncells > 0 && (!cells || !density)) || !npic) return fail(QK_ERR_INVALID, "bad cell list");
+    if (npic[0] <= 0 || npic[1] <= 0 || npic[2] <= 0 || (int64_t)npic[0] * npic[1] * npic[2] > 4096)
+        return fail(QK_ERR_INVALID, "npic must be positive with at most 4096 particles per cell");
+    if (added) *added = 0;
+    // place() keeps the cells inside is_inside (src/plist.cpp:22-31); the ids
+    // of a listed cell are fixed by its index in the list
+    const int npt = npic[0] * npic[1] * npic[2];
+    std::vector<int> kc, kid;
+    std::vector<double> kd;
+    for (int64_t m = 0; m < ncells; ++m) {
+        const int i = cells[3 * m], j = cells[3 * m + 1], k = cells[3 * m + 2];
+        if (!(i > 0 && i < c->p.nx - 2 && j > 0 && j < c->p.ny - 2 && k > 0 && k < c->p.nz - 2)) continue;
+        kc.insert(kc.end(), {i, j, k});
+        kd.push_back(density[m]);
+        kid.push_back(first_id + (int32_t)(m * npt));
+    }
+    const int64_t nk = (int64_t)kd.size();
+    if (nk == 0) return QK_OK;
+    const int64_t total = nk * npt;
+    QK_TRY(ensure_species(c, s, total));
+    const size_t need = (size_t)nk * (3 * sizeof(int) + sizeof(double) + sizeof(int));
+    if (need > c->fill_cap) {
+        if (c->d_fill) HIP_TRY(hipFree(c->d_fill));
+        c->d_fill = nullptr;
+        c->fill_cap = 0;
+        HIP_TRY(hipMalloc(&c->d_fill, need));
+        c->fill_cap = need;
+    }
+    double *dd = (double *)c->d_fill;
+    int *dc = (int *)(dd + nk), *di = dc + 3 * nk;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(dd, kd.data(), (size_t)nk * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc, kc.data(), (size_t)nk * 3 * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di, kid.data(), (size_t)nk * sizeof(int), hipMemcpyHostToDevice));
+    SpeciesBuf &b = c->sp[s];
+    const double sign = b.cmr < 0 ? -1.0 : 1.0;
+    if (c->rsize == 4)
+        HIP_TRY(launch_fill_cells<float>(c->stream, view<float>(b.buf[b.cur]), b.n, nk, dc, dd, di, npic, sign, ux0));
+    else
+        HIP_TRY(launch_fill_cells<double>(c->stream, view<double>(b.buf[b.cur]), b.n, nk, dc, dd, di, npic, sign, ux0));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    b.n += total;
+    if (added) *added = total;
     return QK_OK;
 }
 

@@ -28,7 +28,8 @@ __host__ __device__ inline int cell_j(int c) { return (c >> 10) & 1023; }
 __host__ __device__ inline int cell_k(int c) { return c & 1023; }
 
 // cell tiles of (1 << kTsx) x (1 << kTsy) x (1 << kTsz) cells, numbered
-// i-major; particles are sorted by (tile, cell inside the tile)
+// i-major; particles are sorted by tile (the order inside a tile is the
+// scatter's spread order, see qk_sort.hip)
 #ifndef QK_TILE_SHIFTS
 #define QK_TILE_SHIFTS 3, 3, 3
 #endif
@@ -40,7 +41,8 @@ constexpr int kTileCells = kTx * kTy * kTz;
 struct TileGeom {
     int ntx, nty, ntz;   // tiles per axis
     int ntiles;
-    int nkeys;           // ntiles * kTileCells
+    int nkeys;           // sort keys: one per tile
+    int ishift;          // moving-window shifts since the sort: a tile's cells now start at i = 8 ti - ishift
 };
 
 __host__ __device__ inline int tile_of(const TileGeom &t, int i, int j, int k)
@@ -50,8 +52,17 @@ __host__ __device__ inline int tile_of(const TileGeom &t, int i, int j, int k)
 
 __host__ __device__ inline int sort_key(const TileGeom &t, int c)
 {
-    const int i = c >> 20, j = (c >> 10) & 1023, k = c & 1023;
-    return tile_of(t, i, j, k) * kTileCells + ((((i & (kTx - 1)) << kTsy | (j & (kTy - 1))) << kTsz) | (k & (kTz - 1)));
+    return tile_of(t, c >> 20, (c >> 10) & 1023, c & 1023);
+}
+
+// Workgroups are dealt to the 8 XCDs of the chip round-robin by block index,
+// and each XCD has its own L2.  xcd_block renumbers the blocks of a grid of
+// nb so that XCD x runs the contiguous range [x q + min(x, r), ...) of
+// virtual blocks (q, r = nb / 8, nb % 8): neighbouring work shares one L2.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb)
+{
+    const int64_t q = nb / 8, r = nb % 8, x = b % 8;
+    return x * q + (x < r ? x : r) + b / 8;
 }
 
 // is_inside (src/plist.cpp:42-48)

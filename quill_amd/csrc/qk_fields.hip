@@ -183,34 +183,31 @@ __global__ void __launch_bounds__(kBX * kBY) k_maxwell_e(Grid g, Coef<real> cf, 
 }
 
 // Be = B averaged to E positions (src/advance.cpp:20-83): each component is
-// the mean over a 2x2x2 stencil; on a boundary plane the half that leaves the
-// grid is dropped (4-point mean); cells on two or more boundary planes keep
-// their value.
-template <int C, typename real>
-__device__ inline real be_component(const Grid &g, const real *__restrict__ F, int i, int j, int k)
+// the mean over a 2x2x2 stencil (Bx over i..i+1, j-1..j, k-1..k; By over
+// i-1..i, j..j+1, k-1..k; Bz over i-1..i, j-1..j, k..k+1); on a boundary plane
+// the half that leaves the grid is dropped (4-point mean); cells on two or
+// more boundary planes keep their value.
+//
+// A thread owns one (j, k) column and marches along i over kBeRun planes,
+// carrying the 2x2 transverse sums of the previous plane in registers, so
+// each B value is read about once from L2 and four times from L1.
+constexpr int kBeRun = 16;
+
+// 2x2 sum of F in the (j, k) plane at i: rows j and j + dj, columns k and
+// k + dk, entries outside the grid left out; n counts the entries taken
+template <typename real>
+__device__ inline real plane_sum(const Grid &g, const real *__restrict__ F, int i, int j, int k, int dj, int dk,
+                                 int &n)
 {
-    // stencil offsets along x, y, z for component C
-    const int ox = (C == 0) ? 1 : -1, oy = (C == 1) ? 1 : -1, oz = (C == 2) ? 1 : -1;
-    const int i2 = i + ox, j2 = j + oy, k2 = k + oz;
-    const bool ux = (unsigned)i2 < (unsigned)g.nx, uy = (unsigned)j2 < (unsigned)g.ny, uz = (unsigned)k2 < (unsigned)g.nz;
-    real s = real(0);
-    int cnt = 0;
-    const int ia[2] = {i, i2}, ja[2] = {j, j2}, ka[2] = {k, k2};
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        if (a == 1 && !ux) continue;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            if (b == 1 && !uy) continue;
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (c == 1 && !uz) continue;
-                s += F[ia[a] * g.sx + ja[b] * g.sy + ka[c]];
-                ++cnt;
-            }
-        }
-    }
-    return (cnt == 8 ? real(0.125) : real(0.25)) * s;
+    const int j2 = j + dj, k2 = k + dk;
+    const bool vj = (unsigned)j2 < (unsigned)g.ny, vk = (unsigned)k2 < (unsigned)g.nz;
+    const real *r = F + i * g.sx;
+    real s = r[j * g.sy + k];
+    if (vk) s += r[j * g.sy + k2];
+    if (vj) s += r[j2 * g.sy + k];
+    if (vj && vk) s += r[j2 * g.sy + k2];
+    n = (1 + vj) * (1 + vk);
+    return s;
 }
 
 template <typename real>
@@ -221,14 +218,30 @@ __global__ void __launch_bounds__(kBX * kBY) k_interpolate_be(Grid g, const real
 {
     const int k = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
-    const int i = blockIdx.z;
+    const int i0 = blockIdx.z * kBeRun, i1 = min(i0 + kBeRun, g.nx);
     if (k >= g.nz || j >= g.ny) return;
-    const int nb = (i == 0 || i == g.nx - 1) + (j == 0 || j == g.ny - 1) + (k == 0 || k == g.nz - 1);
-    if (nb >= 2) return;
-    const int id = i * g.sx + j * g.sy + k;
-    bex[id] = be_component<0>(g, bx, i, j, k);
-    bey[id] = be_component<1>(g, by, i, j, k);
-    bez[id] = be_component<2>(g, bz, i, j, k);
+    const int nbjk = (j == 0 || j == g.ny - 1) + (k == 0 || k == g.nz - 1);
+    // entries of each 2x2 transverse sum (the same on every plane)
+    int cx, cy, cz;
+    // x: planes i, i+1; y, z: planes i-1, i
+    real px = plane_sum(g, bx, i0, j, k, -1, -1, cx);
+    real qy = i0 > 0 ? plane_sum(g, by, i0 - 1, j, k, 1, -1, cy) : real(0);
+    real rz = i0 > 0 ? plane_sum(g, bz, i0 - 1, j, k, -1, 1, cz) : real(0);
+    for (int i = i0; i < i1; ++i) {
+        const bool up = i + 1 < g.nx;
+        const real px1 = up ? plane_sum(g, bx, i + 1, j, k, -1, -1, cx) : real(0);
+        const real qy1 = plane_sum(g, by, i, j, k, 1, -1, cy);
+        const real rz1 = plane_sum(g, bz, i, j, k, -1, 1, cz);
+        if (nbjk + (i == 0 || i == g.nx - 1) < 2) {
+            const int id = i * g.sx + j * g.sy + k;
+            bex[id] = (px + px1) * (cx * (1 + up) == 8 ? real(0.125) : real(0.25));
+            bey[id] = (qy + qy1) * (cy * (1 + (i > 0)) == 8 ? real(0.125) : real(0.25));
+            bez[id] = (rz + rz1) * (cz * (1 + (i > 0)) == 8 ? real(0.125) : real(0.25));
+        }
+        px = px1;
+        qy = qy1;
+        rz = rz1;
+    }
 }
 
 // init_boundaries: zero components on boundary planes, transverse indices in
@@ -378,7 +391,8 @@ hipError_t launch_maxwell_e(hipStream_t st, int solver, const Grid &g, const Sol
 template <typename real>
 hipError_t launch_interpolate_be(hipStream_t st, const Grid &g, const Fields<real> &f)
 {
-    k_interpolate_be<real><<<cell_grid(g), dim3(kBX, kBY), 0, st>>>(g, f.bx, f.by, f.bz, f.bex, f.bey, f.bez);
+    const dim3 grid((g.nz + kBX - 1) / kBX, (g.ny + kBY - 1) / kBY, (g.nx + kBeRun - 1) / kBeRun);
+    k_interpolate_be<real><<<grid, dim3(kBX, kBY), 0, st>>>(g, f.bx, f.by, f.bz, f.bex, f.bey, f.bez);
     return hipGetLastError();
 }
 

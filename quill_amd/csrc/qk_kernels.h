@@ -90,6 +90,41 @@ template <typename real>
 hipError_t launch_vacuum_pairs(hipStream_t st, const Grid &g, const Fields<real> &f, int pusher, double q_vac,
                                const QedArgs &q, const AppendTarget<real> &el, const AppendTarget<real> &po);
 
+// ---- macro-particle merging (pmerging, src/mc.cpp:202-462) ---------------
+
+// acc[0] += sum |q|, acc[2] += live count over slots [0, n) (acc: 4 doubles)
+template <typename real>
+hipError_t launch_charge_sum(hipStream_t st, const Species<real> &s, int64_t n, double *acc);
+// mode 1 ("ti"): delete each live particle with probability p, acc[0] +=
+// deleted |q|; mode 2 ("nl"): delete with p / (1 + (2 |q| / av_q)^4), divide
+// the survivors' q by one minus that probability, acc[0] += their new |q|.
+// Draws: Philox keyed by (seed, step, slot, tag).
+template <typename real>
+hipError_t launch_merge_delete(hipStream_t st, const Species<real> &s, int64_t n, int mode, double p, double av_q,
+                               uint64_t seed, uint32_t step, uint32_t tag, double *acc);
+// q *= a for every live particle
+template <typename real>
+hipError_t launch_scale_charge(hipStream_t st, const Species<real> &s, int64_t n, double a);
+
+// ---- moving window (src/advance.cpp:389-454) --------------------------------
+
+// E, B and Be move one plane toward -x (F[i] = F[i + 1]); the plane nx - 1 is zeroed
+template <typename real>
+hipError_t launch_shift_fields(hipStream_t st, const Grid &g, const Fields<real> &f);
+// packed cells of slots [0, n): plane i = 0 is erased, the rest move to i - 1
+hipError_t launch_shift_particles(hipStream_t st, int *cell, int64_t n);
+// *out = max(*out, max over planes [ilo, ihi) of E^2 + B^2) as the bits of a
+// non-negative double (get_max_w, src/low-level.cpp:512-532)
+template <typename real>
+hipError_t launch_max_w(hipStream_t st, const Grid &g, const Fields<real> &f, int ilo, int ihi,
+                        unsigned long long *out);
+// fill_cell_by_particles (src/init.cpp:402-478) with T = 0 on ncells listed
+// cells (i j k triples), npic[0] x npic[1] x npic[2] particles each, written to
+// slots first, first + 1, ...; ids idbase[m] + particle index when s.id is set
+template <typename real>
+hipError_t launch_fill_cells(hipStream_t st, const Species<real> &s, int64_t first, int64_t ncells, const int *cells,
+                             const double *density, const int *idbase, const int npic[3], double sign, double ux0);
+
 // the field-energy sum is spread over this many FP64 slots (one atomic per
 // workgroup lands on slot blockIdx mod kEnergySlots) and summed on the host
 constexpr int kEnergySlots = 256;
@@ -122,8 +157,8 @@ size_t scan_scratch_ints(size_t n);
 hipError_t launch_exclusive_scan(hipStream_t st, const int *in, int *out, int n, int *scratch);
 
 struct SortBuffers {
-    int *counts;      // nkeys + 1
-    int *offsets;     // nkeys + 1 (consumed as cursors by the scatter)
+    int *counts;      // nkeys + 1 (particles per tile)
+    int *offsets;     // nkeys + 1 (first slot per tile; consumed as cursors by the scatter)
     int *tile_start;  // ntiles + 1
     int *tile_mult;   // ntiles: inverse spread multiplier of the tile's storage order
     int *chunk_n;     // ntiles + 1

@@ -28,6 +28,18 @@ namespace {
 #endif
 constexpr int kHalo = QK_HALO;
 constexpr int kT[3] = {kTx, kTy, kTz};
+// threads per workgroup of the tiled kernel: the LDS window and field stage
+// (78 KB) admit two workgroups per CU, so the workgroup is made large enough
+// to keep four wavefronts per SIMD in flight
+#ifndef QK_TBLOCK
+#define QK_TBLOCK 512
+#endif
+#ifdef QK_TWAVES_EU
+#define QK_TATTR __attribute__((amdgpu_waves_per_eu(QK_TWAVES_EU)))
+#else
+#define QK_TATTR
+#endif
+constexpr int kTBlock = QK_TBLOCK;
 
 // Current window, FP64, one block per component.  A deposit from start cell
 // r touches cells r-1 .. r+1 (one face crossing per axis at most) and, for
@@ -107,7 +119,7 @@ __device__ inline void stage_component(real *dst, const real *__restrict__ src, 
                                        int gk0)
 {
     constexpr int d1 = fdim(C, 1), d2 = fdim(C, 2);
-    for (int w = threadIdx.x; w < fsize(C); w += kBlock) {
+    for (int w = threadIdx.x; w < fsize(C); w += kTBlock) {
         const int lk = w % d2, t = w / d2;
         const int lj = t % d1, li = t / d1;
         const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
@@ -123,7 +135,7 @@ __device__ inline void flush_component(real *__restrict__ dst, const double *win
                                        int gk0)
 {
     constexpr int d1 = jdim(C, 1), d2 = jdim(C, 2);
-    for (int w = threadIdx.x; w < jsize(C); w += kBlock) {
+    for (int w = threadIdx.x; w < jsize(C); w += kTBlock) {
         const int wk = w % d2, t = w / d2;
         const int wj = t % d1, wi = t / d1;
         const int gi = gi0 + wi, gj = gj0 + wj, gk = gk0 + wk;
@@ -158,19 +170,21 @@ __global__ void __launch_bounds__(kBlock) k_push_global(Grid g, Fields<real> f, 
 
 
 template <typename real, int PUSHER>
-__global__ void __launch_bounds__(kBlock) k_push_tiled(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
+__global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
                                                        StepConst<real> sc, const int4 *__restrict__ chunks,
                                                        const int *__restrict__ nchunks, TileGeom tg, double *diag)
 {
     __shared__ double win[kJWin];
     __shared__ real stage[2 * kFStage];
-    if ((int)blockIdx.x >= *nchunks) return;   // whole workgroup leaves together
-    const int4 ch = chunks[blockIdx.x];
+    // consecutive chunks (neighbouring tiles) run on one XCD and share its L2
+    const int vb = (int)xcd_block(blockIdx.x, gridDim.x);
+    if (vb >= *nchunks) return;   // whole workgroup leaves together
+    const int4 ch = chunks[vb];
     const int tile = ch.x;
     const int tk = tile % tg.ntz, t2 = tile / tg.ntz;
     const int tj = t2 % tg.nty, ti = t2 / tg.nty;
-    const int i0 = ti * kTx, j0 = tj * kTy, k0 = tk * kTz;
-    for (int w = threadIdx.x; w < kJWin; w += kBlock) win[w] = 0.0;
+    const int i0 = ti * kTx - tg.ishift, j0 = tj * kTy, k0 = tk * kTz;
+    for (int w = threadIdx.x; w < kJWin; w += kTBlock) win[w] = 0.0;
     {
         const int si = i0 - kHalo, sj = j0 - kHalo, sk = k0 - kHalo;
         stage_component<0>(stage, f.ex, g, si, sj, sk);
@@ -204,11 +218,11 @@ __global__ void __launch_bounds__(kBlock) k_push_tiled(Grid g, Fields<real> f, C
     PRec<real> nxt;
     int64_t p = (int64_t)ch.y + threadIdx.x;
     if (QK_PREFETCH && p < end) load_rec(s, p, nxt);
-    for (; p < end; p += kBlock) {
+    for (; p < end; p += kTBlock) {
         PRec<real> r;
         if (QK_PREFETCH) {
             r = nxt;
-            if (p + kBlock < end) load_rec(s, p + kBlock, nxt);
+            if (p + kTBlock < end) load_rec(s, p + kTBlock, nxt);
         } else {
             load_rec(s, p, r);
         }
@@ -247,7 +261,9 @@ __global__ void __launch_bounds__(kBlock) k_photon_move(Grid g, Species<real> s,
     DiagAcc da;
     if (p < first + n) {
         const int c = s.cell[p];
-        if (c >= 0) {
+        if (c >= 0 && !is_inside(g, cell_i(c), cell_j(c), cell_k(c))) {
+            s.cell[p] = -1;   // p_boundary (src/plist.cpp:73-102)
+        } else if (c >= 0) {
             const real u[3] = {s.ux[p], s.uy[p], s.uz[p]};
             const real gam = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
             const real a = sc.dt / gam;
@@ -322,6 +338,58 @@ __global__ void __launch_bounds__(kBlock) k_load_uniform(Grid g, Species<real> s
     if (s.id) s.id[o] = (int)gid;
 }
 
+// pmerging (src/mc.cpp:202-462), per species: total charge and count, the
+// random deletion, the rescale of the survivors
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_charge_sum(Species<real> s, int64_t n, double *acc)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double q = 0, c = 0;
+    if (p < n && s.cell[p] >= 0) {
+        q = fabs((double)s.q[p]);
+        c = 1;
+    }
+    block_accumulate(acc, q, 0.0, c, 0.0);
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_merge_delete(Species<real> s, int64_t n, int mode, double pd, double av_q,
+                                                         uint64_t seed, uint32_t step, uint32_t tag, double *acc)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double sum = 0, c = 0;
+    if (p < n && s.cell[p] >= 0) {
+        const double q = (double)s.q[p], aq = fabs(q);
+        const u32x4 v = philox(u32x4{{(uint32_t)p, step, tag, 0x3E76Eu}}, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double r = ((double)(v.v[0] >> 11) * 4294967296.0 + (double)v.v[1]) * (1.0 / 9007199254740992.0);
+        if (mode == 1) {
+            if (r < pd) {
+                s.cell[p] = -1;
+                sum = aq;
+                c = 1;
+            }
+        } else {
+            const double x = 2 * aq / av_q, pp = pd / (1 + x * x * x * x);   // _ppd, src/mc.cpp:198-200
+            if (r < pp) {
+                s.cell[p] = -1;
+                c = 1;
+            } else {
+                const real qn = (real)(q / (1 - pp));
+                s.q[p] = qn;
+                sum = fabs((double)qn);
+            }
+        }
+    }
+    block_accumulate(acc, sum, 0.0, c, 0.0);
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_scale_charge(Species<real> s, int64_t n, double a)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < n && s.cell[p] >= 0) s.q[p] = (real)((double)s.q[p] * a);
+}
+
 // node charge with trilinear weights (rhodeposition, src/deposition.cpp:358-429)
 template <typename real>
 __global__ void __launch_bounds__(kBlock) k_compute_rho(Grid g, Species<real> s, int64_t n, double *rho, int slot)
@@ -370,9 +438,9 @@ hipError_t launch_push_tiled(hipStream_t st, const Grid &g, const Fields<real> &
     if (a.max_chunks <= 0) return hipSuccess;
     const StepConst<real> sc = step_const<real>(g, a);
     if (a.pusher == 1)
-        k_push_tiled<real, 1><<<a.max_chunks, kBlock, 0, st>>>(g, f, j, s, sc, a.chunks, a.nchunks, a.tg, a.diag);
+        k_push_tiled<real, 1><<<a.max_chunks, kTBlock, 0, st>>>(g, f, j, s, sc, a.chunks, a.nchunks, a.tg, a.diag);
     else
-        k_push_tiled<real, 0><<<a.max_chunks, kBlock, 0, st>>>(g, f, j, s, sc, a.chunks, a.nchunks, a.tg, a.diag);
+        k_push_tiled<real, 0><<<a.max_chunks, kTBlock, 0, st>>>(g, f, j, s, sc, a.chunks, a.nchunks, a.tg, a.diag);
     return hipGetLastError();
 }
 
@@ -394,6 +462,31 @@ hipError_t launch_compute_rho(hipStream_t st, const Grid &g, const Species<real>
     return hipGetLastError();
 }
 
+template <typename real>
+hipError_t launch_charge_sum(hipStream_t st, const Species<real> &s, int64_t n, double *acc)
+{
+    if (n <= 0) return hipSuccess;
+    k_charge_sum<real><<<blocks_for(n), kBlock, 0, st>>>(s, n, acc);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_merge_delete(hipStream_t st, const Species<real> &s, int64_t n, int mode, double p, double av_q,
+                               uint64_t seed, uint32_t step, uint32_t tag, double *acc)
+{
+    if (n <= 0) return hipSuccess;
+    k_merge_delete<real><<<blocks_for(n), kBlock, 0, st>>>(s, n, mode, p, av_q, seed, step, tag, acc);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_scale_charge(hipStream_t st, const Species<real> &s, int64_t n, double a)
+{
+    if (n <= 0) return hipSuccess;
+    k_scale_charge<real><<<blocks_for(n), kBlock, 0, st>>>(s, n, a);
+    return hipGetLastError();
+}
+
 #define QK_INST(real)                                                                                            \
     template hipError_t launch_push_global<real>(hipStream_t, const Grid &, const Fields<real> &,               \
                                                  const Current<real> &, const Species<real> &, const PushArgs &); \
@@ -402,7 +495,11 @@ hipError_t launch_compute_rho(hipStream_t st, const Grid &g, const Species<real>
     template hipError_t launch_load_uniform<real>(hipStream_t, const Grid &, const Species<real> &, int64_t, int, \
                                                   double, double, uint64_t, int, int);                           \
     template hipError_t launch_compute_rho<real>(hipStream_t, const Grid &, const Species<real> &, int64_t,       \
-                                                 double *, int);
+                                                 double *, int);                                                  \
+    template hipError_t launch_charge_sum<real>(hipStream_t, const Species<real> &, int64_t, double *);           \
+    template hipError_t launch_merge_delete<real>(hipStream_t, const Species<real> &, int64_t, int, double, double, \
+                                                  uint64_t, uint32_t, uint32_t, double *);                        \
+    template hipError_t launch_scale_charge<real>(hipStream_t, const Species<real> &, int64_t, double);
 QK_INST(float)
 QK_INST(double)
 #undef QK_INST

@@ -57,7 +57,7 @@ struct GlobalAcc {
 // Every thread of the workgroup must call it; calls may follow each other.
 __device__ inline void block_accumulate(double *dst, double v0, double v1, double v2, double v3)
 {
-    __shared__ double part[4][kBlock / 64];
+    __shared__ double part[4][16];   // up to 1024 threads
     v0 = wave_sum(v0);
     v1 = wave_sum(v1);
     v2 = wave_sum(v2);
@@ -73,7 +73,7 @@ __device__ inline void block_accumulate(double *dst, double v0, double v1, doubl
     __syncthreads();
     if (threadIdx.x == 0) {
         double s[4] = {0, 0, 0, 0};
-        for (int m = 0; m < kBlock / 64; ++m)
+        for (int m = 0; m < (int)(blockDim.x >> 6); ++m)
             for (int a = 0; a < 4; ++a) s[a] += part[a][m];
         if (s[2] != 0 || s[3] != 0)
             for (int a = 0; a < 4; ++a) atomicAdd(dst + a, s[a]);
@@ -156,6 +156,13 @@ __device__ inline void advance_particle(const Grid &g, const Corners<real> &node
     const int c = r.cell;
     if (c < 0) return;
     int i = cell_i(c), j = cell_j(c), k = cell_k(c);
+    // p_boundary (src/plist.cpp:73-102) erases the layers 0, n-2, n-1 before
+    // the push; only a moving-window shift leaves particles there (place()
+    // keeps is_inside cells only)
+    if (!is_inside(g, i, j, k)) {
+        r.cell = -1;
+        return;
+    }
     // local copies stay in registers (the deposit indexes them by axis)
     real pos[3] = {r.o[0], r.o[1], r.o[2]};
     real u[3] = {r.u[0], r.u[1], r.u[2]};

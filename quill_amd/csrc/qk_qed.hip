@@ -81,7 +81,11 @@ __global__ void __launch_bounds__(kBlock) k_push_qed_lepton(Grid g, Fields<real>
 {
     const int64_t p = first + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     DiagAcc da, dph;
-    const int cell = p < first + n ? s.cell[p] : -1;
+    int cell = p < first + n ? s.cell[p] : -1;
+    if (cell >= 0 && !is_inside(g, cell_i(cell), cell_j(cell), cell_k(cell))) {
+        s.cell[p] = -1;   // p_boundary (src/plist.cpp:73-102)
+        cell = -1;
+    }
     if (cell >= 0) {
         int i = cell_i(cell), j = cell_j(cell), k = cell_k(cell);
         real pos[3] = {s.ox[p], s.oy[p], s.oz[p]};
@@ -197,7 +201,11 @@ __global__ void __launch_bounds__(kBlock) k_photon_qed(Grid g, Fields<real> f, C
 {
     const int64_t p = first + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     DiagAcc da, de, dp;
-    const int cell = p < first + n ? s.cell[p] : -1;
+    int cell = p < first + n ? s.cell[p] : -1;
+    if (cell >= 0 && !is_inside(g, cell_i(cell), cell_j(cell), cell_k(cell))) {
+        s.cell[p] = -1;   // p_boundary (src/plist.cpp:73-102)
+        cell = -1;
+    }
     if (cell >= 0) {
         int i = cell_i(cell), j = cell_j(cell), k = cell_k(cell);
         real pos[3] = {s.ox[p], s.oy[p], s.oz[p]};

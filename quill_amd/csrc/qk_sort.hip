@@ -2,12 +2,18 @@
 // (replaces the per-cell linked lists and place() re-linking of
 // src/plist.cpp:7-40 and the page allocator of src/low-level.cpp:186-286).
 //
-//   1. count live particles per key (tile, cell in tile)
-//   2. exclusive scan of the counts -> first slot of every key
-//   3. scatter every live particle to its key's next slot in the other buffer
-//      (free slots are dropped: this is also the compaction)
-//   4. per tile: first slot and the number of kChunk-particle chunks, scanned
-//      into the work list of the tiled push kernel
+//   1. count live particles per cell tile
+//   2. exclusive scan of the counts -> first slot of every tile
+//   3. per tile: the number of kChunk-particle chunks and the spread
+//      multiplier of its storage order
+//   4. scatter every live particle to the next position of its tile in the
+//      other buffer, stored at that tile's spread slot (free slots are
+//      dropped: this is also the compaction)
+//   5. chunk counts scanned into the work list of the tiled push kernel
+//
+// Particles arrive in the order of the previous sort, so the wavefronts of
+// the count and scatter passes see long runs of one tile and need about one
+// atomic per run.
 #include "qk_kernels.h"
 
 namespace qk {
@@ -95,12 +101,16 @@ __global__ void __launch_bounds__(256) k_count_keys(const int *__restrict__ cell
     if (lane == head && key >= 0) atomicAdd(counts + key, len);
 }
 
+// The scatter reads its source in XCD-sized contiguous ranges (xcd_block):
+// the particles of one tile, contiguous in the source, are then written by
+// one XCD, whose L2 merges their scattered stores into whole lines instead
+// of collecting partial lines in eight L2s.
 template <typename real>
 __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real> dst, int64_t n, TileGeom tg,
                                                  int *cursor, const int *__restrict__ tile_start,
                                                  const int *__restrict__ tile_mult)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int c = p < n ? src.cell[p] : -1;
     const int key = c >= 0 ? sort_key(tg, c) : -1;
     const int lane = threadIdx.x & 63;
@@ -111,7 +121,7 @@ __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real
     base = __shfl(base, head, 64);
     if (c < 0) return;
     // sorted position inside the tile -> its storage slot
-    const int t = key / kTileCells;
+    const int t = key;
     const int ts = tile_start[t], nt = tile_start[t + 1] - ts;
     const int o = ts + (int)((int64_t)(base + (lane - head) - ts) * tile_mult[t] % nt);
     dst.cell[o] = c;
@@ -126,13 +136,15 @@ __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real
     if (src.id) dst.id[o] = src.id[p];
 }
 
-// Storage order inside a tile.  The n particles of a tile, sorted by cell,
-// are stored so that slot s holds sorted position (s * a) mod n, with a the
+// Storage order inside a tile.  The n particles of a tile, in arrival order,
+// are stored so that slot s holds arrival position (s * a) mod n, with a the
 // first of kSpread coprime to n.  A wavefront reading 64 consecutive slots
-// then gets particles a positions apart in cell order, i.e. from distinct
-// cells while a cell holds fewer than a particles: its LDS atomics in the
-// push hit distinct addresses, and its memory accesses stay contiguous.
-// The scatter applies the inverse map, position e -> slot e * a^-1 mod n.
+// then gets particles a arrival positions apart; particles that arrive
+// grouped by cell (a loader writes a cell's particles together) land in
+// distinct cells, and already spread ones stay spread.  The push kernel's
+// LDS atomics then hit distinct addresses while its memory accesses stay
+// contiguous.  The scatter applies the inverse map, position e -> slot
+// e * a^-1 mod n.
 constexpr int kSpread[8] = {37, 41, 43, 47, 53, 59, 61, 67};
 
 // a^-1 mod n for gcd(a, n) = 1 (extended Euclid); 0 for n = 1
@@ -156,10 +168,10 @@ __global__ void k_tile_chunks(const int *__restrict__ offsets, int ntiles, int *
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > ntiles) return;
-    const int s = offsets[t * kTileCells];
+    const int s = offsets[t];
     tile_start[t] = s;
     if (t < ntiles) {
-        const int cnt = offsets[(t + 1) * kTileCells] - s;
+        const int cnt = offsets[t + 1] - s;
         chunk_n[t] = (cnt + kChunk - 1) / kChunk;
         int m = 1;   // identity if every spread shares a factor with cnt
         for (int a : kSpread)

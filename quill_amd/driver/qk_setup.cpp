@@ -4,6 +4,7 @@
 // conversions with the laser wavelength in cm).
 #include "qk_setup.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -58,21 +59,30 @@ bool find_boolean(const Ctx &c, const char *name, bool dflt)
     return dflt;
 }
 
-// the last iteration of OutputIterations (src/parameters.cpp:6-38)
-int last_iteration(const std::vector<double> &t_end, const std::vector<double> &period, double dt)
+// the output iterations and times of OutputIterations (src/parameters.cpp:6-38)
+void output_iterations(const std::vector<double> &t_end, const std::vector<double> &period, double dt,
+                       std::vector<int> &iters, std::vector<double> &times)
 {
     int previous = -1;
     double t_begin = 0;
     for (size_t n = 0; n < t_end.size(); ++n) {
         const int count = (int)((t_end[n] + 0.5 * dt - t_begin) / period[n]) + 1;
         for (int j = 0; j < count; ++j) {
-            const int it = (int)std::round((t_begin + j * period[n]) / dt);
-            if (it > previous) previous = it;
+            const double time = t_begin + j * period[n];
+            const int it = (int)std::round(time / dt);
+            if (it > previous) {
+                iters.push_back(it);
+                times.push_back(time);
+                previous = it;
+            }
         }
         t_begin = t_end[n];
     }
     const int final_it = (int)std::round(t_end.back() / dt);
-    return final_it > previous ? final_it : previous;
+    if (final_it > previous) {
+        iters.push_back(final_it);
+        times.push_back(t_end.back());
+    }
 }
 
 std::vector<std::string> split_amp(const std::string &s)
@@ -149,7 +159,8 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
             t_end.push_back(te);
             period.push_back(op);
         } while (c.get("t_end", from).value != 0);
-        run.last_iteration = last_iteration(t_end, period, dt);
+        output_iterations(t_end, period, dt, run.out_iter, run.out_time);
+        run.last_iteration = run.out_iter.back();
     }
 
     // laser (src/main.cpp:2681-2905)
@@ -219,14 +230,63 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
         }
     }
 
-    // options outside this library's per-step path are refused
+    // moving window (src/main.cpp:2665-2679, 2932-2958): on unless "off"
     {
         const ConfVar &mw = c.get("mwindow");
-        const bool flows = c.get("nelflow").value != 0 || c.get("nerflow").value != 0;
-        if (mw.units != "off" && !flows) {
-            err = "the moving window is not supported; set mwindow = off";
+        run.mwindow = mw.units == "off" ? 0 : (mw.units == "auto" ? 2 : 1);
+        run.mwtolerance = c.get("mwtolerance").value;
+        if (run.mwtolerance == 0) run.mwtolerance = 1e-6;
+        run.mwspeed = c.get("mwspeed").value;
+        if (run.mwindow == 1 && run.mwspeed == 0) run.mwspeed = 1;
+        run.mwseed = c.get("mwseed").units != "off";
+        run.mwseed_ions = c.get("mwseed_ions").units == "on";
+        double mw_mcr = c.get("mw_mcr").value;
+        if (mw_mcr == 0) mw_mcr = 1;
+        run.mw_cmr = 1 / (kProtonMass * mw_mcr);
+        const ConfVar &ta = c.get("t_add_mw");
+        double tv = ta.value;
+        if (ta.units == "um") tv = tv * 1e-4 / c.lambda;
+        else if (ta.units == "mm") tv = tv / (10 * c.lambda);
+        else if (ta.units == "cm") tv = tv / c.lambda;
+        run.t_add_mw = tv != 0 ? tv * 2 * kPi : run.out_time.back();
+        run.ne_ncr = 1 / (k0 * k0);
+        run.ylength = ylength;
+        run.zlength = zlength;
+        // find_array(first, name, "1/k", "lambda") / (name, "") (src/main.cpp:3496-3525)
+        auto coords = [&](const char *name, std::vector<double> &out) {
+            const ConfVar &v = c.get(name);
+            ConfVar one = v;
+            out.clear();
+            for (double a : v.array) {
+                one.value = a;
+                double x;
+                if (!time_to_k(c, one, "lambda", x, err)) return false;
+                out.push_back(x);
+            }
+            return true;
+        };
+        if (!coords("ne_profile_x_coords", run.ne_x_coords) || !coords("ne_profile_r_coords", run.ne_r_coords))
             return false;
+        run.ne_x_values = c.get("ne_profile_x_values").array;
+        run.ne_r_values = c.get("ne_profile_r_values").array;
+        for (int a = 0; a < 2; ++a) {
+            const std::vector<double> &xs = a ? run.ne_r_coords : run.ne_x_coords;
+            const std::vector<double> &vs = a ? run.ne_r_values : run.ne_x_values;
+            if (xs.size() != vs.size()) {
+                err = std::string("the sizes of ne_profile_") + (a ? "r" : "x") + "_coords and _values differ";
+                return false;
+            }
+            for (size_t m = 1; m < xs.size(); ++m)
+                if (xs[m] < xs[m - 1]) {
+                    err = std::string("ne_profile_") + (a ? "r" : "x") + "_coords is not sorted";
+                    return false;
+                }
         }
+    }
+
+    // options outside this library's per-step path are refused
+    {
+        const bool flows = c.get("nelflow").value != 0 || c.get("nerflow").value != 0;
         if (flows) {
             err = "neutral flows (nelflow / nerflow) are not supported";
             return false;
@@ -236,8 +296,8 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
             return false;
         }
         const std::string &pm = c.get("pmerging").units;
-        if (!pm.empty() && pm != "off") {
-            err = "particle merging (pmerging = " + pm + ") is not supported";
+        if (!pm.empty() && pm != "off" && pm != "ti" && pm != "nl") {
+            err = "pmerging = " + pm + " is not a merging algorithm (ti, nl)";
             return false;
         }
         if (c.get("catching").units == "on" || c.get("dump_photons").units == "on") {
@@ -291,7 +351,7 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
             f.ne_y1 = n1 * to_ncr;
             const double mcr = c.get("mcr", from).value == 0 ? 1 : c.get("mcr", from).value;
             f.cmr = 1 / (kProtonMass * mcr);
-            f.ions = ions == "on";
+            f.ions = ions == "on" ? 1 : (ions == "positrons" ? 2 : 0);
             f.T = c.get("Tfilm", from).value;
             f.vx = c.get("vxfilm", from).value;
             f.npic[0] = (int)c.get("xnpic_film", from).value;
@@ -312,9 +372,20 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
             for (double m : mcrs) seen = seen || m == *it;
             if (!seen) mcrs.push_back(*it);
         }
+        // the ions of mwseed_ions (src/main.cpp:3310-3317)
+        if (run.mwseed_ions) {
+            double mw_mcr = c.get("mw_mcr").value;
+            if (mw_mcr == 0) mw_mcr = 1;
+            bool seen = false;
+            for (double m : mcrs) seen = seen || m == mw_mcr;
+            if (!seen) mcrs.push_back(mw_mcr);
+        }
         run.ion_mcr = (ions == "on" || ions == "positrons") ? mcrs : std::vector<double>{};
     }
     const int xnpic = (int)c.get("xnpic").value, ynpic = (int)c.get("ynpic").value, znpic = (int)c.get("znpic").value;
+    run.npic[0] = xnpic;
+    run.npic[1] = ynpic;
+    run.npic[2] = znpic;
     for (Film &f : run.films)
         if (f.npic[0] == 0 || f.npic[1] == 0 || f.npic[2] == 0) {
             f.npic[0] = xnpic;
@@ -326,6 +397,27 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
         return false;
     }
     run.data_folder = c.get("data_folder").units.empty() ? "results" : c.get("data_folder").units;
+
+    // what the output iterations write (src/main.cpp:2720-2731, 2954-2972, 3387-3395, 3551-3555)
+    {
+        auto comps = [&](const char *name, bool out[3]) {
+            const std::string v = c.get(name).units;
+            out[0] = v == "x" || v == "xy" || v == "xz" || v == "xyz";
+            out[1] = v == "y" || v == "xy" || v == "yz" || v == "xyz";
+            out[2] = v == "z" || v == "xz" || v == "yz" || v == "xyz";
+        };
+        comps("e_components_for_output", run.out_set.e);
+        comps("b_components_for_output", run.out_set.b);
+        const std::string jc = c.get("j_components_for_output").units;
+        for (int a = 0; a < 3; ++a) run.out_set.j[a] = jc.find("xyz"[a]) != std::string::npos && jc != "none";
+        const std::string pf = c.get("particles_for_output").units;
+        run.out_set.rho_p = pf == "ep" || pf == "epph";
+        run.out_set.rho_ph = pf == "eph" || pf == "epph";
+        double x0fout = length(c, c.get("x0fout"), true);
+        if (x0fout == 0) x0fout = xlength / 2;
+        run.out_grid.i_yz = (int)((xlength - x0fout) / dx);
+        run.out_grid.binary = c.get("output_mode").units == "binary";
+    }
 
     // physics options (src/main.cpp:3531-3598)
     p.qed = find_boolean(c, "qed", true) ? 1 : 0;
@@ -351,6 +443,13 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
     p.nx = (int)(xlength / dx);
     p.ny = (int)(ylength / dy);
     p.nz = (int)(zlength / dz);
+    run.out_grid.nx = p.nx;
+    run.out_grid.ny = p.ny;
+    run.out_grid.nz = p.nz;
+    if (run.out_grid.i_yz < 0 || run.out_grid.i_yz >= p.nx) {
+        err = "x0fout puts the yz output plane outside the box";
+        return false;
+    }
     p.dx = dx;
     p.dy = dy;
     p.dz = dz;
@@ -365,6 +464,16 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
     p.precision = QK_FP32;
     p.deposit = QK_DEPOSIT_TILED;
     p.sort_every = 20;
+    // merging and its limit crnp = crpc x the box in cells (src/main.cpp:1699-1701, 3397-3401)
+    {
+        const std::string &pm = c.get("pmerging").units;
+        p.merging = pm == "ti" ? 1 : (pm == "nl" ? 2 : 0);
+        p.crnp = c.get("crpc").value * xlength * ylength * zlength / (dx * dy * dz);
+        if (p.merging && !(p.crnp > 0)) {
+            err = "pmerging needs crpc > 0";
+            return false;
+        }
+    }
 
     // laser calls of init_fields, cos envelope (src/main.cpp:1150-1227)
     auto call = [&](double ay, double az, double xc, double xs, double y, double z, double append, double ph) {
@@ -423,20 +532,23 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
     return true;
 }
 
-void film_particles(const Run &run, const Film &f, double cmr, ParticleSet &out)
+void film_cells(const Run &run, const Film &f, double x0, double x1, bool column, std::vector<int32_t> &ijk,
+                std::vector<double> &density)
 {
-    // film() (src/init.cpp:537-646) with T = 0 and no transverse profile
     const qk_params &p = run.params;
     const double dx = p.dx, dy = p.dy, dz = p.dz;
-    int i0 = (int)(f.x0 / dx), i1 = (int)(f.x1 / dx);
-    i0 = std::min(std::max(i0, 0), p.nx);
-    i1 = std::min(std::max(i1, 0), p.nx);
+    int i0 = (int)(x0 / dx), i1 = (int)(x1 / dx);
+    if (column) {
+        // append_for_moving_window: only the column i = nx - 3, when the film reaches it
+        i0 = p.nx - 3;
+        i1 = (i0 < p.nx - 2 && i1 >= p.nx - 2) ? p.nx - 2 : i0;
+    } else {
+        i0 = std::min(std::max(i0, 0), p.nx);
+        i1 = std::min(std::max(i1, 0), p.nx);
+    }
     const int jy0 = (int)(f.y0 / dy), jy1 = (int)(f.y1 / dy);
     const int kz0 = (int)(f.z0 / dz), kz1 = (int)(f.z1 / dz);
     const double film_ylen = double(jy1 - jy0 - 3);
-    const double ux0 = f.vx / std::sqrt(1 - f.vx * f.vx);
-    const int bi = f.npic[0], bj = f.npic[1], bk = f.npic[2];
-    const bool electron = cmr < 0;
     for (int i = i0; i < i1; ++i)
         for (int j = jy0 + 1; j < jy1 - 1; ++j) {
             double nes = f.ne_y1 * (j - jy0 - 1) / film_ylen + f.ne_y0 * (jy1 - 2 - j) / film_ylen;
@@ -445,24 +557,108 @@ void film_particles(const Run &run, const Film &f, double cmr, ParticleSet &out)
                 if (f.gradwidth_y >= 0 && j - jy0 - 1 < f.gradwidth_y / dy) nes *= (j - jy0 - 1) / (f.gradwidth_y / dy);
                 if (f.gradwidth_y >= 0 && jy1 - 2 - j < f.gradwidth_y / dy) nes *= (jy1 - 2 - j) / (f.gradwidth_y / dy);
                 double n;
-                if (f.gradwidth >= 0) n = i >= (int)((f.x0 + f.gradwidth) / dx) ? nes : nes * (i * dx - f.x0) / f.gradwidth;
-                else n = i >= (int)((f.x0 - f.gradwidth) / dx) ? nes : nes * (1 + (i * dx - f.x0) / f.gradwidth);
-                // fill_cell_by_particles (src/init.cpp:402-478), dsplmt = 0
-                double q0 = 1 / double(bi * bj * bk) * n;
-                if (electron) q0 = -q0;
-                for (int ip = 0; ip < bi; ++ip)
-                    for (int jp = 0; jp < bj; ++jp)
-                        for (int kp = 0; kp < bk; ++kp) {
-                            out.x.push_back(0.5 / bi + double(i) + double(ip) / double(bi));
-                            out.y.push_back(0.5 / bj + double(j) + double(jp) / double(bj));
-                            out.z.push_back(0.5 / bk + double(k) + double(kp) / double(bk));
-                            out.ux.push_back(ux0);
-                            out.uy.push_back(0);
-                            out.uz.push_back(0);
-                            out.q.push_back(q0);
-                        }
+                if (f.gradwidth >= 0) n = i >= (int)((x0 + f.gradwidth) / dx) ? nes : nes * (i * dx - x0) / f.gradwidth;
+                else n = i >= (int)((x0 - f.gradwidth) / dx) ? nes : nes * (1 + (i * dx - x0) / f.gradwidth);
+                ijk.insert(ijk.end(), {i, j, k});
+                density.push_back(n);
             }
         }
+}
+
+void film_particles(const Run &run, const Film &f, double cmr, ParticleSet &out)
+{
+    // film() (src/init.cpp:537-646) with T = 0: fill_cell_by_particles
+    // (src/init.cpp:402-478, dsplmt = 0) on each of its cells
+    std::vector<int32_t> ijk;
+    std::vector<double> dens;
+    film_cells(run, f, f.x0, f.x1, false, ijk, dens);
+    const double ux0 = f.vx / std::sqrt(1 - f.vx * f.vx);
+    const int bi = f.npic[0], bj = f.npic[1], bk = f.npic[2];
+    for (size_t m = 0; m < dens.size(); ++m) {
+        const int i = ijk[3 * m], j = ijk[3 * m + 1], k = ijk[3 * m + 2];
+        double q0 = 1 / double(bi * bj * bk) * dens[m];
+        if (cmr < 0) q0 = -q0;
+        for (int ip = 0; ip < bi; ++ip)
+            for (int jp = 0; jp < bj; ++jp)
+                for (int kp = 0; kp < bk; ++kp) {
+                    out.x.push_back(0.5 / bi + double(i) + double(ip) / double(bi));
+                    out.y.push_back(0.5 / bj + double(j) + double(jp) / double(bj));
+                    out.z.push_back(0.5 / bk + double(k) + double(kp) / double(bk));
+                    out.ux.push_back(ux0);
+                    out.uy.push_back(0);
+                    out.uz.push_back(0);
+                    out.q.push_back(q0);
+                }
+    }
+}
+
+double lin_interpolation(double x, const std::vector<double> &xs, const std::vector<double> &vs)
+{
+    if (xs.empty()) return 1.0;
+    if (x < xs.front()) return vs.front();
+    if (x > xs.back()) return vs.back();
+    const size_t i = std::lower_bound(xs.begin(), xs.end(), x) - xs.begin();
+    // x == xs[0]: the reference reads vs[-1] here; the limit from the right is vs[0]
+    if (i == 0) return vs.front();
+    const double left = vs[i - 1], right = vs[i];
+    return left + (right - left) * (x - xs[i - 1]) / (xs[i] - xs[i - 1]);
+}
+
+namespace {
+
+int ion_slot(const Run &run, double cmr)
+{
+    for (int n = 0; n < run.params.n_ion; ++n)
+        if (run.params.ion_cmr[n] == cmr) return QK_ION0 + n;
+    return -1;
+}
+
+}  // namespace
+
+std::vector<CellFill> moving_window_fills(const Run &run, int nmw)
+{
+    const qk_params &p = run.params;
+    std::vector<CellFill> out;
+    // the seed column at i = nx - 3 (src/main.cpp:1756-1823)
+    auto seed = [&](int species) {
+        CellFill f{species, {}, {}, {run.npic[0], run.npic[1], run.npic[2]}, 0.0};
+        double n;
+        n = run.ne_ncr;
+        n *= lin_interpolation((nmw - 1) * p.dx, run.ne_x_coords, run.ne_x_values);
+        for (int j = 0; j < int(run.ylength / p.dy); j++)
+            for (int k = 0; k < int(run.zlength / p.dz); k++) {
+                const double zcell = k * p.dz, ycell = j * p.dy;
+                const double ycenter = run.ylength / 2.0, zcenter = run.zlength / 2.0;
+                const double zrel = zcell - zcenter, yrel = ycell - ycenter;
+                const double r = std::sqrt(zrel * zrel + yrel * yrel);
+                const double modifier = lin_interpolation(r, run.ne_r_coords, run.ne_r_values);
+                if (modifier != 0.0) {
+                    f.ijk.insert(f.ijk.end(), {p.nx - 3, j, k});
+                    f.density.push_back(n * modifier);
+                }
+            }
+        if (!f.density.empty()) out.push_back(std::move(f));
+    };
+    if (run.mwseed) seed(QK_ELECTRON);
+    if (run.mwseed_ions) {
+        const int s = ion_slot(run, run.mw_cmr);
+        if (s >= 0) seed(s);
+    }
+    // every film's column, newest film first (src/main.cpp:1822-1833)
+    for (const Film &f : run.films) {
+        const double ux0 = f.vx / std::sqrt(1 - f.vx * f.vx);
+        CellFill e{QK_ELECTRON, {}, {}, {f.npic[0], f.npic[1], f.npic[2]}, ux0};
+        film_cells(run, f, f.x0 - p.dx * nmw, f.x1 - p.dx * nmw, true, e.ijk, e.density);
+        if (e.density.empty()) continue;
+        if (f.ions == 1) {
+            const int s = ion_slot(run, f.cmr);
+            if (s >= 0) out.push_back(CellFill{s, e.ijk, e.density, {f.npic[0], f.npic[1], f.npic[2]}, ux0});
+        } else if (f.ions == 2) {
+            out.push_back(CellFill{QK_POSITRON, e.ijk, e.density, {f.npic[0], f.npic[1], f.npic[2]}, ux0});
+        }
+        out.push_back(std::move(e));
+    }
+    return out;
 }
 
 }  // namespace qkd

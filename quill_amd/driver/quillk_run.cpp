@@ -8,11 +8,15 @@
 //   quillk-run CONF [--steps N] [--fp64] [--deposit tiled|global]
 //              [--sort-every N] [--device D] [--out DIR] [--dry-run]
 //
+// Output iterations write the reference's plane files (ex .. bz, w, inv,
+// rho, rho_p, rho_ph, jx .. jz as the config asks) unless --no-output.
 // --dry-run prints the derived run as JSON and touches no GPU;
 // --print-conf prints the parsed variables as JSON; --dump-setup DIR writes
 // the initial particles of every species (raw little-endian doubles, rows
 // of x y z ux uy uz q, file species<slot>.f64) and the laser calls
-// (lasers.f64, rows of 16) for an independent run of the same set-up.
+// (lasers.f64, rows of 16) for an independent run of the same set-up;
+// --mw-fills NMW prints the moving window's column fills after the shift
+// that made nmw (species, cells, densities) as JSON.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -24,6 +28,7 @@
 #include <vector>
 
 #include "qk_conf.h"
+#include "qk_output.h"
 #include "qk_setup.h"
 #include "quillk.h"
 
@@ -31,11 +36,15 @@ using namespace qkd;
 
 namespace {
 
+constexpr double kPi = 3.14159265358979323846;   // include/physical_constants.h:3
+
 int usage()
 {
     std::fprintf(stderr,
                  "usage: quillk-run CONF [--steps N] [--fp64] [--deposit tiled|global] [--sort-every N]\n"
-                 "                  [--device D] [--out DIR] [--dry-run] [--print-conf] [--dump-setup DIR]\n");
+                 "                  [--device D] [--out DIR] [--no-output] [--dry-run] [--print-conf]\n"
+                 "                  [--dump-setup DIR] [--mw-fills NMW]\n"
+                 "       quillk-run --write-planes IN DIR\n");
     return 2;
 }
 
@@ -58,11 +67,13 @@ std::vector<Species> make_particles(const Run &run)
 {
     std::vector<Species> out;
     out.push_back({QK_ELECTRON, -1, {}});
+    out.push_back({QK_POSITRON, 1, {}});
     for (size_t n = 0; n < run.ion_mcr.size(); ++n) out.push_back({QK_ION0 + (int)n, run.params.ion_cmr[n], {}});
     for (const Film &f : run.films) {
         film_particles(run, f, -1, out[0].set);
-        if (f.ions)
-            for (size_t n = 1; n < out.size(); ++n)
+        if (f.ions == 2) film_particles(run, f, 1, out[1].set);
+        if (f.ions == 1)
+            for (size_t n = 2; n < out.size(); ++n)
                 if (out[n].cmr == f.cmr) film_particles(run, f, f.cmr, out[n].set);
     }
     return out;
@@ -77,12 +88,34 @@ void print_json(const Run &run, const std::vector<Species> &sp, int steps)
                 solver_name(p.solver), p.pusher == QK_PUSHER_VAY ? "vay" : "boris", p.qed, run.lambda_cm);
     std::printf(" \"e_s\": %.17g, \"q_vac\": %.17g, \"external_bz\": %.17g,\n", p.e_s, p.q_vac, p.external_bz);
     std::printf(" \"norm_n\": %.17g, \"norm_ef\": %.17g, \"norm_ep\": %.17g,\n", p.norm_n, p.norm_ef, p.norm_ep);
+    std::printf(" \"merging\": %d, \"crnp\": %.17g, \"outputs\": %zu,\n", p.merging, p.crnp, run.out_iter.size());
+    std::printf(" \"mwindow\": %d, \"mwspeed\": %.17g, \"mwtolerance\": %.17g, \"t_add_mw\": %.17g, \"mw_fills\": [",
+                run.mwindow, run.mwspeed, run.mwtolerance, run.t_add_mw);
+    // the column fills after the first shift (nmw = 2): species, cells, sum of densities
+    const std::vector<CellFill> fills = moving_window_fills(run, 2);
+    for (size_t n = 0; n < fills.size(); ++n) {
+        double sum = 0;
+        for (double d : fills[n].density) sum += d;
+        std::printf("%s[%d, %zu, %.17g]", n ? ", " : "", fills[n].species, fills[n].density.size(), sum);
+    }
+    std::printf("],\n");
     std::printf(" \"ion_cmr\": [");
     for (int n = 0; n < p.n_ion; ++n) std::printf("%s%.17g", n ? ", " : "", p.ion_cmr[n]);
     std::printf("],\n \"lasers\": [");
     for (size_t n = 0; n < run.lasers.size(); ++n) {
         std::printf("%s[", n ? ", " : "");
         for (int m = 0; m < 16; ++m) std::printf("%s%.17g", m ? ", " : "", run.lasers[n].L[m]);
+        std::printf("]");
+    }
+    // film() arguments of the set-up, newest first (src/init.cpp:537-541):
+    // x0 x1 ne_y0 ne_y1 ions cmr gradwidth y0 y1 z0 z1 T vx xnpic ynpic znpic gradwidth_y
+    std::printf("],\n \"film_args\": [");
+    for (size_t n = 0; n < run.films.size(); ++n) {
+        const Film &f = run.films[n];
+        const double a[17] = {f.x0, f.x1, f.ne_y0, f.ne_y1, (double)f.ions, f.cmr, f.gradwidth, f.y0, f.y1, f.z0,
+                              f.z1, f.T, f.vx, (double)f.npic[0], (double)f.npic[1], (double)f.npic[2], f.gradwidth_y};
+        std::printf("%s[", n ? ", " : "");
+        for (int m = 0; m < 17; ++m) std::printf("%s%.17g", m ? ", " : "", a[m]);
         std::printf("]");
     }
     std::printf("],\n \"films\": %zu, \"particles\": {", run.films.size());
@@ -92,6 +125,25 @@ void print_json(const Run &run, const std::vector<Species> &sp, int steps)
         std::printf("%s\"%d\": {\"n\": %zu, \"q_sum\": %.17g}", n ? ", " : "", sp[n].slot, sp[n].set.q.size(), qsum);
     }
     std::printf("}}\n");
+}
+
+// --mw-fills NMW: the column fills of add_moving_window_particles after the
+// shift that made nmw, as JSON
+void print_fills(const Run &run, int nmw)
+{
+    const std::vector<CellFill> fills = moving_window_fills(run, nmw);
+    std::printf("[");
+    for (size_t n = 0; n < fills.size(); ++n) {
+        const CellFill &f = fills[n];
+        std::printf("%s\n {\"species\": %d, \"npic\": [%d, %d, %d], \"ux0\": %.17g, \"cells\": [", n ? "," : "",
+                    f.species, f.npic[0], f.npic[1], f.npic[2], f.ux0);
+        for (size_t m = 0; m < f.density.size(); ++m)
+            std::printf("%s[%d, %d, %d]", m ? ", " : "", f.ijk[3 * m], f.ijk[3 * m + 1], f.ijk[3 * m + 2]);
+        std::printf("], \"density\": [");
+        for (size_t m = 0; m < f.density.size(); ++m) std::printf("%s%.17g", m ? ", " : "", f.density[m]);
+        std::printf("]}");
+    }
+    std::printf("\n]\n");
 }
 
 void print_conf(const std::vector<ConfVar> &vars)
@@ -123,14 +175,36 @@ bool dump_setup(const std::string &dir, const Run &run, const std::vector<Specie
     return (bool)f;
 }
 
+// --write-planes IN DIR: the writers on given arrays (a check of the file
+// formats against the reference's own writers).  IN holds int32 nx ny nz
+// i_yz binary, then E, B, Be and rho as double[nx][ny][nz][3].
+int write_planes(const char *in, const char *dir)
+{
+    std::ifstream f(in, std::ios::binary);
+    int32_t h[5];
+    if (!f.read((char *)h, sizeof h)) return 1;
+    OutputGrid g{h[0], h[1], h[2], h[3], h[4] != 0};
+    const size_t n = 3 * (size_t)g.nx * g.ny * g.nz;
+    std::vector<double> E(n), B(n), Be(n), rho(n);
+    for (std::vector<double> *a : {&E, &B, &Be, &rho})
+        if (!f.read((char *)a->data(), (std::streamsize)(n * sizeof(double)))) return 1;
+    OutputSet s;
+    for (int c = 0; c < 3; ++c) s.e[c] = s.b[c] = true;
+    s.rho_p = s.rho_ph = true;
+    mkdir(dir, 0755);
+    write_output_set(dir, "", g, s, E.data(), B.data(), Be.data(), rho.data(), nullptr, 1.0);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     if (argc < 2) return usage();
+    if (argc == 4 && std::strcmp(argv[1], "--write-planes") == 0) return write_planes(argv[2], argv[3]);
     std::string conf = argv[1], out_dir, setup_dir;
-    int steps_override = -1, device = 0, sort_every = -1;
-    bool fp64 = false, dry = false, global_dep = false, show_conf = false;
+    int steps_override = -1, device = 0, sort_every = -1, fills_nmw = 0;
+    bool fp64 = false, dry = false, global_dep = false, show_conf = false, write_outputs = true;
     for (int a = 2; a < argc; ++a) {
         const std::string s = argv[a];
         auto next = [&]() -> const char * { return a + 1 < argc ? argv[++a] : nullptr; };
@@ -138,11 +212,13 @@ int main(int argc, char **argv)
         else if (s == "--fp64") fp64 = true;
         else if (s == "--dry-run") dry = true;
         else if (s == "--print-conf") show_conf = true;
+        else if (s == "--no-output") write_outputs = false;
         else if (s == "--dump-setup") { const char *v = next(); if (!v) return usage(); setup_dir = v; }
         else if (s == "--deposit") { const char *v = next(); if (!v) return usage(); global_dep = std::strcmp(v, "global") == 0; }
         else if (s == "--sort-every") { const char *v = next(); if (!v) return usage(); sort_every = std::atoi(v); }
         else if (s == "--device") { const char *v = next(); if (!v) return usage(); device = std::atoi(v); }
         else if (s == "--out") { const char *v = next(); if (!v) return usage(); out_dir = v; }
+        else if (s == "--mw-fills") { const char *v = next(); if (!v) return usage(); fills_nmw = std::atoi(v); }
         else return usage();
     }
 
@@ -176,6 +252,10 @@ int main(int argc, char **argv)
         print_json(run, sp, steps);
         return 0;
     }
+    if (fills_nmw > 0) {
+        print_fills(run, fills_nmw);
+        return 0;
+    }
     if (out_dir.empty()) out_dir = run.data_folder;
     mkdir(out_dir.c_str(), 0755);
 
@@ -193,13 +273,42 @@ int main(int argc, char **argv)
     }
     if (!rc) rc = check(qk_init_boundaries(ctx), "qk_init_boundaries");
 
-    std::ofstream fN(out_dir + "/N"), fE(out_dir + "/energy");
+    std::ofstream fN(out_dir + "/N"), fE(out_dir + "/energy"), fMW;
+    if (run.mwindow) fMW.open(out_dir + "/mwcoordinate");
     const int batch = 100;
     std::vector<qk_diag> diag(batch);
     double seconds = 0;
     int64_t pushed = 0;
+    size_t next_out = 0;   // output iterations are written after their step (src/main.cpp:2344-2430)
+    const size_t nf = 3 * (size_t)p.nx * p.ny * p.nz;
+    int nmw = 1;           // src/main.cpp:122
+    // check_moving_window (src/main.cpp:1953-1978) after step l; nm = 4
+    auto window_moves = [&](int l, bool &moves) -> int {
+        moves = false;
+        if (run.mwindow == 1) {
+            moves = (l + 1) * p.dt * run.mwspeed > nmw * p.dx;
+        } else if (run.mwindow == 2) {
+            double max_w = 0, front = 0;
+            int r = check(qk_max_w(ctx, 0, p.nx, &max_w), "qk_max_w");
+            if (!r) r = check(qk_max_w(ctx, p.nx - 4 - 2, p.nx - 4, &front), "qk_max_w");
+            if (r) return r;
+            const double ratio = max_w > 0 ? front / max_w : 0.0;
+            moves = ratio > run.mwtolerance;
+        }
+        return 0;
+    };
     for (int done = 0; !rc && done < steps;) {
-        const int nb = std::min(batch, steps - done);
+        int nb = std::min(batch, steps - done);
+        if (write_outputs && next_out < run.out_iter.size()) nb = std::min(nb, run.out_iter[next_out] + 1 - done);
+        // the window is checked after every step: "on" batches up to the next
+        // shift (known in advance), "auto" steps one at a time
+        if (run.mwindow == 2) nb = 1;
+        if (run.mwindow == 1)
+            for (int l = done; l < done + nb; ++l)
+                if ((l + 1) * p.dt * run.mwspeed > nmw * p.dx) {
+                    nb = l - done + 1;
+                    break;
+                }
         const auto t0 = std::chrono::steady_clock::now();
         rc = check(qk_step(ctx, nb, diag.data()), "qk_step");
         seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -211,7 +320,37 @@ int main(int argc, char **argv)
             fE << std::endl;
             pushed += d.n_e + d.n_p + d.n_ph + d.n_ion;
         }
+        // moving_window after the batch's last step (src/main.cpp:2333-2337);
+        // write_mwcoordinate each step (:154-158, 2381-2383)
+        bool moves = false;
+        const int last = done + nb - 1;
+        if (!rc && run.mwindow) {
+            rc = window_moves(last, moves);
+            if (!rc && moves) {
+                rc = check(qk_moving_window(ctx), "qk_moving_window");
+                ++nmw;
+            }
+            for (int l = 0; l < nb - 1; ++l) fMW << ((nmw - (moves ? 2 : 1)) * p.dx / 2 / kPi) << std::endl;
+            fMW << ((nmw - 1) * p.dx / 2 / kPi) << std::endl;
+        }
         done += nb;
+        if (!rc && write_outputs && next_out < run.out_iter.size() && done == run.out_iter[next_out] + 1) {
+            std::vector<double> E(nf), B(nf), J(nf), Be(nf), rho(nf);
+            rc = check(qk_get_fields(ctx, E.data(), B.data(), J.data(), Be.data()), "qk_get_fields");
+            if (!rc) rc = check(qk_compute_rho(ctx, rho.data()), "qk_compute_rho");
+            if (!rc)
+                write_output_set(out_dir, iteration_suffix(run.out_time[next_out]), run.out_grid, run.out_set,
+                                 E.data(), B.data(), Be.data(), rho.data(), J.data(), p.dx / p.dt);
+            ++next_out;
+        }
+        // add_moving_window_particles (src/main.cpp:1750-1835, 2396-2398)
+        if (!rc && moves && (last + 1) * p.dt < run.t_add_mw)
+            for (const CellFill &f : moving_window_fills(run, nmw)) {
+                rc = check(qk_fill_cells(ctx, f.species, (int64_t)f.density.size(), f.ijk.data(), f.density.data(),
+                                         f.npic, f.ux0, 0, nullptr),
+                           "qk_fill_cells");
+                if (rc) break;
+            }
     }
     if (!rc)
         std::printf("{\"steps\": %d, \"seconds\": %.6f, \"ms_per_step\": %.6f, \"particle_steps_per_s\": %.6g}\n", steps,

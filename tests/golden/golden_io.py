@@ -10,7 +10,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 MAGIC = 0x4C4C5551
-MODE_RUN, MODE_LASER, MODE_RATES, MODE_PUSH, MODE_TIME = 0, 1, 2, 3, 4
+MODE_RUN, MODE_LASER, MODE_RATES, MODE_PUSH, MODE_TIME, MODE_FOUT, MODE_MERGE, MODE_MW, MODE_FILM = range(9)
 DIAG_WIDTH = 12
 DIAG_NAMES = ["N_e", "N_p", "N_ph", "n_e", "n_p", "n_ph", "energy_f", "energy_e", "energy_p",
               "energy_ph", "energy_ion", "n_ion"]
@@ -57,6 +57,22 @@ def read_output_run(path, h):
         ids = np.frombuffer(f.read(n * 4), "<i4")
         diag = np.frombuffer(f.read(int(h["nsteps"]) * DIAG_WIDTH * 8), "<f8").reshape(-1, DIAG_WIDTH)
     return fields, rec, ids, diag
+
+
+def read_output_mw(path, h):
+    """RUN output followed by the per-step (nmw, ratio) trace."""
+    nx, ny, nz = int(h["nx"]), int(h["ny"]), int(h["nz"])
+    nf = nx * ny * nz * 3
+    nsteps = int(h["nsteps"])
+    with open(path, "rb") as f:
+        f.read(HDR.itemsize)
+        fields = [np.frombuffer(f.read(nf * 8), "<f8").reshape(nx, ny, nz, 3) for _ in range(4)]
+        n = int(np.frombuffer(f.read(8), "<i8")[0])
+        rec = np.frombuffer(f.read(n * 80), "<f8").reshape(n, 10)
+        ids = np.frombuffer(f.read(n * 4), "<i4")
+        diag = np.frombuffer(f.read(nsteps * DIAG_WIDTH * 8), "<f8").reshape(-1, DIAG_WIDTH)
+        trace = np.frombuffer(f.read(nsteps * 16), "<f8").reshape(-1, 2)
+    return fields, rec, ids, diag, trace
 
 
 def read_output_fields(path, h):

@@ -15,6 +15,14 @@ Cases
   laser_cos  f_init_cos (src/init.cpp:6-210), sscos = 0, tilted and rotated
   rates      chi / w / tilde_w / mathcal_W (src/mc.cpp:26-116) on a table
   push       push_boris / push_vay (src/pusher.cpp) on a table
+  fout       the reference's plane writers (ex .. bz, w, inv, rho, rho_p,
+             rho_ph; src/fout.cpp:180-660) in text and binary output mode
+  merge_stats  pmerging "ti" and "nl" (src/mc.cpp:202-462) over 16 seeds of the
+             reference's random stream (statistical parity)
+  mw_on, mw_auto  a moving window (src/advance.cpp:389-454) with "on" and
+             "auto" triggers and column fills (tests/golden/mw_case.py)
+  film       film() on the films quillk-run derives from the texts of
+             tests/golden/film_case.py, initial and moving-window columns
   qed_stats  QED on (photon emission, Breit-Wheeler): per-step diagnostics and
              the photon-gamma histogram of 16 runs with distinct seeds of the
              reference's own random stream (statistical parity)
@@ -32,6 +40,9 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
 
 import golden_io as gio  # noqa: E402
+import film_case  # noqa: E402
+import merge_case  # noqa: E402
+import mw_case  # noqa: E402
 import qed_case  # noqa: E402
 import oracle  # noqa: E402
 
@@ -182,6 +193,138 @@ def case_qed_stats():
     print(f"qed_stats: {qed_case.SEEDS} seeds, final photons {s[:, 0].mean():.0f}, positrons {s[:, 1].mean():.1f}")
 
 
+def case_merge_stats():
+    """pmerging of the reference (src/mc.cpp:202-462), "ti" and "nl", over
+    merge_case.SEEDS seeds of its own random stream: run summaries."""
+    c = merge_case.inputs()
+    out = {}
+    for mode in ("ti", "nl"):
+        summ = []
+        for seed in range(1, merge_case.SEEDS + 1):
+            h = gio.header(mode=gio.MODE_MERGE, nx=c["nx"], ny=c["ny"], nz=c["nz"], n_particles=len(c["rec"]),
+                           seed=seed, n_ion=1, dx=0.3, dy=0.3, dz=0.3, dt=0.1, e_s=E_S)
+            h["ion_cmr"][0] = merge_case.ION_CMR
+            h["laser"][:4] = merge_case.PPD[mode]
+            h["extra"][0] = 1 if mode == "ti" else 2
+            with tempfile.TemporaryDirectory() as td:
+                fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+                gio.write_input(fi, h, c["rec"], c["ids"])
+                subprocess.run([oracle.REF_BIN, fi, fo], check=True, stdout=subprocess.DEVNULL)
+                with open(fo, "rb") as f:
+                    f.read(gio.HDR.itemsize)
+                    n = int(np.frombuffer(f.read(8), "<i8")[0])
+                    rec = np.frombuffer(f.read(n * 80), "<f8").reshape(n, 10)
+            summ.append(merge_case.summary(rec))
+        out[mode] = np.array(summ)
+    np.savez_compressed(os.path.join(HERE, "merge_stats.npz"), **out)
+    print(f"merge_stats: {merge_case.SEEDS} seeds, ti electrons kept {out['ti'][:, 0].mean():.0f}, "
+          f"nl electrons kept {out['nl'][:, 0].mean():.0f}")
+
+
+def case_mw(mode):
+    """A run with a moving window (tests/golden/mw_case.py) by the reference
+    harness (mode 7): fields, particles by id, diagnostics and the per-step
+    (nmw, ratio) trace."""
+    c = mw_case.inputs(mode)
+    nx, ny, nz, dx, dt = c["nx"], c["ny"], c["nz"], c["dx"], c["dt"]
+    solver, pusher = (oracle.FDTD, oracle.BORIS) if mode == "on" else (oracle.NDFX, oracle.VAY)
+    h = gio.header(mode=gio.MODE_MW, nx=nx, ny=ny, nz=nz, solver=solver, pusher=pusher, qed=0,
+                   nsteps=c["nsteps"], n_particles=len(c["rec"]), seed=1, n_ion=1, dx=dx, dy=dx, dz=dx, dt=dt,
+                   e_s=E_S, **norms(dx, dx, dx))
+    h["ion_cmr"][0] = mw_case.ION_CMR
+    h["extra"][:] = [c["mw"], c["mwspeed"], c["tol"], c["density"], *c["npic"], c["t_add"]]
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        gio.write_input(fi, h, c["E"], c["B"], c["rec"], c["ids"])
+        subprocess.run([oracle.REF_BIN, fi, fo], check=True, stdout=subprocess.DEVNULL)
+        (Eo, Bo, Jo, BEo), prec, pids, diag, trace = gio.read_output_mw(fo, h)
+    order = np.argsort(pids, kind="stable")
+    np.savez_compressed(os.path.join(HERE, f"mw_{mode}.npz"), header=h.tobytes(), E_in=c["E"], B_in=c["B"],
+                        part_in=c["rec"], ids_in=c["ids"], E=Eo, B=Bo, J=Jo, Be=BEo, part=prec[order],
+                        ids=pids[order], diag=diag, trace=trace)
+    print(f"mw_{mode}: {len(c['rec'])} particles in, {len(prec)} out, shifts {int(trace[-1, 0]) - 1}, "
+          f"ratios {np.round(trace[:, 1], 4).tolist()}")
+
+
+def case_film():
+    """film() (src/init.cpp:537-646) of the reference with the arguments
+    quillk-run derives from the texts of tests/golden/film_case.py: each
+    film's initial particles, and for the window text the column film()
+    fills at i = nx - 3 after nmw shifts (append_for_moving_window)."""
+    import json
+    drv = os.path.join(ROOT, "quill_amd", "bin", "quillk-run")
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, text in film_case.CONFS.items():
+            conf = os.path.join(td, name + ".conf")
+            with open(conf, "w") as f:
+                f.write(text)
+            j = json.loads(subprocess.run([drv, conf, "--dry-run"], check=True, capture_output=True,
+                                          text=True).stdout)
+            calls = []
+            for k, a in enumerate(j["film_args"]):
+                calls.append((f"{name}_init_{k}", a, 0))
+                if name == "window":
+                    for nmw in film_case.NMW:
+                        b = list(a)
+                        b[0] -= j["dx"] * nmw
+                        b[1] -= j["dx"] * nmw
+                        calls.append((f"{name}_col_{nmw}_{k}", b, 1))
+            for key, a, append in calls:
+                h = gio.header(mode=gio.MODE_FILM, nx=j["nx"], ny=j["ny"], nz=j["nz"], dx=j["dx"], dy=j["dy"],
+                               dz=j["dz"], dt=j["dt"], e_s=E_S)
+                h["laser"][:] = a[:16]
+                h["extra"][0] = append
+                h["extra"][1] = a[16]
+                fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+                gio.write_input(fi, h)
+                subprocess.run([oracle.REF_BIN, fi, fo], check=True, stdout=subprocess.DEVNULL)
+                with open(fo, "rb") as f:
+                    f.read(gio.HDR.itemsize)
+                    n = int(np.frombuffer(f.read(8), "<i8")[0])
+                    out[key] = np.frombuffer(f.read(n * 80), "<f8").reshape(n, 10).copy()
+                out[key + "_args"] = np.array(a)
+    np.savez_compressed(os.path.join(HERE, "film.npz"), **out)
+    print("film: " + ", ".join(f"{k} {len(v)}" for k, v in out.items() if not k.endswith("_args")))
+
+
+FOUT_FILES = ["ex", "ey", "ez", "bx", "by", "bz", "w", "inv", "rho", "rho_p", "rho_ph"]
+
+
+def case_fout():
+    """The reference's own plane writers (fout_ex .. fout_bz, fout_w, fout_inv,
+    fout_rho with their yz-plane parts) on random fields and a mixed particle
+    population, in text and in binary output mode."""
+    nx, ny, nz, iyz = 14, 12, 10, 6
+    dx = 0.05 * 2 * PI
+    rng = np.random.default_rng(55)
+    E = rng.uniform(-0.7, 0.7, (nx, ny, nz, 3))
+    B = rng.uniform(-0.7, 0.7, (nx, ny, nz, 3))
+    rec, ids, icmr = mixed_particles(nx, ny, nz, 55, 0.3)
+    out = {"E": E, "B": B, "i_yz": np.int32(iyz)}
+    for binary in (0, 1):
+        h = gio.header(mode=gio.MODE_FOUT, nx=nx, ny=ny, nz=nz, n_particles=len(rec), n_ion=1, dx=dx, dy=dx, dz=dx,
+                       dt=0.5 * dx, e_s=E_S)
+        h["ion_cmr"][0] = icmr
+        h["extra"][0] = iyz
+        h["extra"][1] = binary
+        with tempfile.TemporaryDirectory() as td:
+            fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out")
+            gio.write_input(fi, h, E, B, rec, ids.astype(np.int32))
+            subprocess.run([oracle.REF_BIN, fi, fo], check=True, stdout=subprocess.DEVNULL)
+            with open(fo, "rb") as f:
+                f.read(gio.HDR.itemsize)
+                nf = nx * ny * nz * 3
+                _, _, rho, be = [np.frombuffer(f.read(nf * 8), "<f8").reshape(nx, ny, nz, 3) for _ in range(4)]
+            out["rho"], out["Be"] = rho, be
+            tag = "bin" if binary else "text"
+            for name in FOUT_FILES:
+                with open(fo + "." + name, "rb") as f:
+                    out[f"{tag}_{name}"] = np.frombuffer(f.read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "fout.npz"), **out)
+    print(f"fout: {len(FOUT_FILES)} files x 2 modes")
+
+
 def main(names):
     """Generate the named fixtures (all when none are named)."""
     if not os.path.exists(oracle.REF_BIN):
@@ -201,6 +344,11 @@ def main(names):
         "rates": case_rates,
         "push": case_push,
         "qed_stats": case_qed_stats,
+        "fout": case_fout,
+        "merge_stats": case_merge_stats,
+        "mw_on": lambda: case_mw("on"),
+        "mw_auto": lambda: case_mw("auto"),
+        "film": case_film,
     }
     for name in names or list(cases):
         cases[name]()

@@ -151,8 +151,50 @@ def test_reflections_and_circular(tmp_path):
     assert a[1, 10] == pytest.approx(-a[0, 10])       # xy mirror flips z0
 
 
-@pytest.mark.parametrize("extra, msg", [("mwindow = on\n", "moving window"), ("beam = on\n", "beam"),
-                                        ("pmerging = ti\n", "merging"), ("f_envelope = focused\n", "focused"),
+def test_merging_options(tmp_path):
+    j = json.loads(run(tmp_path, "pmerging = nl\ncrpc = 2.5\n" + BASE, "--dry-run").stdout)
+    # crnp = crpc * xlength ylength zlength / (dx dy dz), src/main.cpp:1701
+    dx, dy = f32(0.05) * TWO_PI, f32(0.1) * TWO_PI
+    cells = (f32(3.2001) * TWO_PI) * (f32(2.4001) * TWO_PI) ** 2 / (dx * dy * dy)
+    assert j["merging"] == 2 and j["crnp"] == pytest.approx(f32(2.5) * cells, rel=1e-12)
+    # output iterations t = 0, 0.25, 0.5
+    assert j["outputs"] == 3
+
+
+def test_moving_window_options(tmp_path):
+    # mwindow is on unless "off" (src/main.cpp:2932-2947), mwspeed 1, t_add_mw = t_end
+    j = json.loads(run(tmp_path, BASE.replace("mwindow = off\n", ""), "--dry-run").stdout)
+    assert (j["mwindow"], j["mwspeed"], j["mwtolerance"]) == (1, 1.0, 1e-6)
+    assert j["t_add_mw"] == pytest.approx(0.5 * TWO_PI, rel=1e-15)
+    # seed column (src/main.cpp:1756-1786): every (j, k) below int(length / d),
+    # density 1 / k0^2 = ne / n_cr; the film (x0film = 1, width 1) does not
+    # reach the column nx - 3 = 61 after one shift
+    assert j["mw_fills"] == [[0, 24 * 24, pytest.approx(24 * 24 * f32(0.01), rel=1e-12)]]
+    # "auto", mwseed off, a film reaching the front: its column only,
+    # cells j, k in [1, 23) (film(), src/init.cpp:548-560, 588-592)
+    text = ("mwindow = auto\nmwtolerance = 0.01\nmwseed = off\nt_add_mw = 3 um\n" +
+            BASE.replace("mwindow = off\n", "").replace("filmwidth = 1", "filmwidth = 5"))
+    j = json.loads(run(tmp_path, text, "--dry-run").stdout)
+    assert (j["mwindow"], j["mwtolerance"]) == (2, f32(0.01))
+    assert j["t_add_mw"] == pytest.approx(f32(3) * 1e-4 / 1e-4 * TWO_PI, rel=1e-12)
+    assert j["mw_fills"] == [[0, 22 * 22, pytest.approx(22 * 22 * f32(0.01), rel=1e-6)]]
+    # density profiles (lin_interpolation, src/low-level.cpp:492-510): x at
+    # (nmw - 1) dx = dx, r from the box centre
+    text = ("ne_profile_x_coords = [0;10]\nne_profile_x_values = [1;3]\n"
+            "ne_profile_r_coords = [0;0.5]\nne_profile_r_values = [1;0]\n" + BASE.replace("mwindow = off\n", ""))
+    j = json.loads(run(tmp_path, text, "--dry-run").stdout)
+    dx, dy = f32(0.05) * TWO_PI, f32(0.1) * TWO_PI
+    nx_ = 1 + 2 * dx / (10 * TWO_PI)
+    L = f32(2.4001) * TWO_PI
+    jj, kk = np.meshgrid(np.arange(24), np.arange(24), indexing="ij")
+    r = np.hypot(jj * dy - L / 2, kk * dy - L / 2)
+    mod = np.where(r > 0.5 * TWO_PI, 0.0, 1 - r / (0.5 * TWO_PI))
+    assert j["mw_fills"][0][:2] == [0, int((mod != 0).sum())]
+    assert j["mw_fills"][0][2] == pytest.approx(f32(0.01) * nx_ * mod.sum(), rel=1e-9)
+
+
+@pytest.mark.parametrize("extra, msg", [("beam = on\n", "beam"),
+                                        ("pmerging = xx\n", "merging"), ("f_envelope = focused\n", "focused"),
                                         ("solver = spectral\n", "solver unknown")])
 def test_unsupported_options_are_refused(tmp_path, extra, msg):
     # find() takes the first occurrence, so the override goes first

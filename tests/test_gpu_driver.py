@@ -102,3 +102,106 @@ def test_driver_run_matches_oracle(tmp_path, solver, pusher):
     np.testing.assert_allclose(E[:, 0], E_ref[:, 0], rtol=1e-4)
     np.testing.assert_allclose(E[:, 1], E_ref[:, 1], rtol=1e-3, atol=1e-4 * E_ref[:, 1].max())
     np.testing.assert_allclose(E[:, 4], E_ref[:, 4], rtol=1e-3, atol=1e-4 * E_ref[:, 4].max())
+
+    # output iterations t = 0, 0.25, 0.5 lambda/c: plane files w, inv, rho
+    # (the formats themselves are checked against the reference's writers in
+    # test_output_writers.py); rho of the last one against the oracle
+    nx, ny, nz = j["nx"], j["ny"], j["nz"]
+    lines = nx * (ny + nz) + ny * nz
+    for t in ("0", "0.25", "0.5"):
+        for name in ("w", "inv", "rho"):
+            path = out / (name + t)
+            assert path.exists(), path
+            assert len(path.read_text().splitlines()) == lines
+    rho_gpu = np.loadtxt(out / "rho0.5")
+    rho_ref = o.compute_rho()[..., 0]
+    iyz = int((float(np.float32(2.4001)) * 2 * np.pi / 2) / j["dx"])
+    expect = np.concatenate([np.concatenate([rho_ref[i, :, nz // 2], rho_ref[i, ny // 2, :]]) for i in range(nx)] +
+                            [rho_ref[iyz].ravel()])
+    np.testing.assert_allclose(rho_gpu, expect, rtol=2e-5, atol=1e-4 * np.abs(expect).max())
+
+
+MW_CONF = """\
+dx = 0.05
+dy = 0.1
+dz = 0.1
+dt = 0.025
+lambda = 1 um
+ne = 0.05 ncr
+xlength = 2.4001
+ylength = 1.2001
+zlength = 1.2001
+t_end = 0.5
+t_add_mw = 0.4
+output_period = 0.5
+f_envelope = cos
+polarization = linear
+a0 = 1
+xsigma = 0.5
+ysigma = 0.4
+zsigma = 0.4
+x0 = 0.4
+mwindow = on
+ne_profile_x_coords = [0;1]
+ne_profile_x_values = [1;2]
+qed = off
+solver = ndfx
+pusher = vay
+xnpic = 1
+ynpic = 2
+znpic = 2
+"""
+
+
+@pytest.mark.gpu
+def test_driver_moving_window_matches_oracle(tmp_path):
+    """mwindow = on with a seeded plasma column (src/main.cpp:1750-1786) and
+    no initial plasma: the driver's N, energy and mwcoordinate records
+    against the oracle stepped with the same shifts (check_moving_window,
+    src/main.cpp:1958-1960) and the column fills the driver reports
+    (--mw-fills; their cells and densities are pinned against the reference
+    in test_driver.py and test_driver_films.py)."""
+    if not os.path.exists(BIN):
+        pytest.fail("quillk-run not built")
+    conf = tmp_path / "mw.conf"
+    conf.write_text(MW_CONF)
+    j = json.loads(subprocess.run([BIN, str(conf), "--dry-run"], capture_output=True, text=True, timeout=60).stdout)
+    print(f"driver moving window: {j['nx']}x{j['ny']}x{j['nz']}, {j['steps']} steps", flush=True)
+    out = tmp_path / "out"
+    r = subprocess.run([BIN, str(conf), "--out", str(out), "--no-output"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    steps, dx, dt = j["steps"], j["dx"], j["dt"]
+    o = oracle.Oracle(j["nx"], j["ny"], j["nz"], dx, j["dy"], j["dz"], dt, solver=oracle.NDFX, pusher=oracle.VAY,
+                      norm_n=j["norm_n"], norm_ef=j["norm_ef"], norm_ep=j["norm_ep"])
+    for L in j["lasers"]:
+        o.laser_cos(*L[:6], b_sign=L[6], x1=L[7], phase=L[8], y0=L[9], z0=L[10], append=bool(L[11]), phi=L[12],
+                    xtarget=L[13], ytarget=L[14], ztarget=L[15])
+    o.init_boundaries()
+    nmw, ref, mwc, fills = 1, [], [], 0
+    for l in range(steps):
+        ref.append(o.step(1, diag=True)[0])
+        moves = (l + 1) * dt * j["mwspeed"] > nmw * dx
+        if moves:
+            o.moving_window()
+            nmw += 1
+        mwc.append((nmw - 1) * dx / 2 / np.pi)
+        if moves and (l + 1) * dt < j["t_add_mw"]:
+            col = json.loads(subprocess.run([BIN, str(conf), "--mw-fills", str(nmw)], capture_output=True, text=True,
+                                            timeout=60).stdout)
+            for f in col:
+                assert f["species"] == 0
+                o.fill_cells(-1.0, f["cells"], f["density"], f["npic"], f["ux0"])
+            fills += 1
+    assert nmw > 8 and fills > 5
+    N = np.loadtxt(out / "N").reshape(-1, 3)
+    E = np.loadtxt(out / "energy").reshape(-1, 4)
+    W = np.loadtxt(out / "mwcoordinate")
+    assert N.shape[0] == steps and E.shape[0] == steps and W.shape[0] == steps
+    np.testing.assert_allclose(W, mwc, rtol=1e-5, atol=1e-12)
+    N_ref = np.array([[d.N_e, d.N_p, d.N_ph] for d in ref])
+    assert N_ref[-1, 0] > 0
+    np.testing.assert_allclose(N, N_ref, rtol=2e-5)
+    E_ref = np.array([[d.energy_f, d.energy_e, d.energy_p, d.energy_ph] for d in ref])
+    np.testing.assert_allclose(E[:, 0], E_ref[:, 0], rtol=1e-4)
+    np.testing.assert_allclose(E[:, 1], E_ref[:, 1], rtol=1e-3, atol=1e-4 * E_ref[:, 1].max())

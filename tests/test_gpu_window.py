@@ -1,0 +1,184 @@
+"""The moving window on the GPU (HIP path through the C ABI).
+
+  * golden parity: the cases of tests/golden/mw_case.py ("on" and "auto"
+    triggers, column fills) against the compiled reference's runs
+    (mw_on.npz, mw_auto.npz): FP64 with global deposit to 1e-10, FP32 with
+    both deposits to FP32 tolerance; the shift sequence must be identical
+  * qk_moving_window alone on a larger grid: the shift is a copy, so E, B,
+    Be and the particles must move bit-exactly
+  * qk_max_w against numpy on the same (rounded) fields
+  * the tiled deposit keeps its tiles across shifts (no re-sort): same
+    result as the global deposit
+"""
+import numpy as np
+import pytest
+
+import golden_io as gio
+import mw_case
+import quill_amd as qa
+
+pytestmark = pytest.mark.gpu
+
+SOLVER = {0: "fdtd", 1: "ndfx", 2: "five_point"}
+PUSHER = {0: "vay", 1: "boris"}
+
+
+def hdr(fx):
+    return np.frombuffer(fx["header"].tobytes(), gio.HDR)[0]
+
+
+def region_of(h, precision, deposit, **kw):
+    return qa.Region(int(h["nx"]), int(h["ny"]), int(h["nz"]), float(h["dx"]), float(h["dy"]), float(h["dz"]),
+                     float(h["dt"]), solver=SOLVER[int(h["solver"])], pusher=PUSHER[int(h["pusher"])],
+                     precision=precision, deposit=deposit, track_ids=True, e_s=float(h["e_s"]),
+                     ion_cmr=[float(h["ion_cmr"][0])], norm_n=float(h["norm_n"]), norm_ef=float(h["norm_ef"]),
+                     norm_ep=float(h["norm_ep"]), **kw)
+
+
+def all_particles(r):
+    recs, ids = zip(*[r.particles(s) for s in range(qa.ION0 + 1)])
+    rec, idv = np.concatenate(recs), np.concatenate(ids)
+    order = np.argsort(idv, kind="stable")
+    return rec[order], idv[order]
+
+
+def run_mw(fx, mode, precision, deposit, c=None, **kw):
+    h = hdr(fx)
+    c = c or mw_case.inputs(mode)
+    print(f"moving window {mode} precision={precision} deposit={deposit}", flush=True)
+    with region_of(h, precision, deposit, **kw) as r:
+        r.set_fields(fx["E_in"], fx["B_in"])
+        r.add_records(fx["part_in"], fx["ids_in"])
+        r.init_boundaries()
+        diag, trace = mw_case.drive(r, c, lambda cells, dens, npic, first: r.fill_cells(qa.ELECTRON, cells, dens,
+                                                                                       npic, 0.0, first))
+        E, B, J, Be = r.get_fields()
+        rec, ids = all_particles(r)
+        fb = r.fallback_fraction()
+    return dict(E=E, B=B, J=J, Be=Be, part=rec, ids=ids, diag=diag, trace=trace, fallback=fb)
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / den if den > 0 else np.linalg.norm(a)
+
+
+@pytest.mark.parametrize("mode", ["on", "auto"])
+def test_fp64_window_matches_reference(mode):
+    fx = gio.load(f"mw_{mode}")
+    out = run_mw(fx, mode, "fp64", "global")
+    np.testing.assert_array_equal(out["trace"][:, 0], fx["trace"][:, 0])
+    np.testing.assert_allclose(out["trace"][:, 1], fx["trace"][:, 1], rtol=1e-10)
+    for key in ("E", "B", "J", "Be"):
+        err = np.abs(out[key] - fx[key]).max() / np.abs(fx[key]).max()
+        assert err < 1e-10, f"mw_{mode} {key}: max rel err {err:.3e}"
+    np.testing.assert_array_equal(out["ids"], fx["ids"])
+    np.testing.assert_allclose(out["part"][:, :3], fx["part"][:, :3], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(out["part"][:, 3:6], fx["part"][:, 3:6], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(out["part"][:, 7], fx["part"][:, 7], rtol=1e-14)
+    for l, d in enumerate(out["diag"]):
+        np.testing.assert_allclose([d[k] for k in gio.DIAG_NAMES], fx["diag"][l], rtol=1e-10, atol=0,
+                                   err_msg=f"mw_{mode} step {l}")
+
+
+@pytest.mark.parametrize("deposit", ["global", "tiled"])
+@pytest.mark.parametrize("mode", ["on", "auto"])
+def test_fp32_window_matches_reference(mode, deposit):
+    fx = gio.load(f"mw_{mode}")
+    # the auto trigger compares a ratio with mwtolerance: keep FP32 away from ties
+    if mode == "auto":
+        tol = mw_case.inputs(mode)["tol"]
+        assert np.abs(fx["trace"][:, 1] - tol).min() > 0.1 * tol
+    out = run_mw(fx, mode, "fp32", deposit)
+    np.testing.assert_array_equal(out["trace"][:, 0], fx["trace"][:, 0])
+    for key in ("E", "B", "J", "Be"):
+        for c in range(3):
+            err = rel_l2(out[key][..., c], fx[key][..., c])
+            assert err < 1e-4, f"mw_{mode} {key}[{c}]: rel L2 {err:.3e}"
+    common, ia, ib = np.intersect1d(out["ids"], fx["ids"], return_indices=True)
+    assert len(common) >= 0.995 * len(fx["ids"])
+    a, b = out["part"][ia], fx["part"][ib]
+    np.testing.assert_allclose(a[:, :3], b[:, :3], rtol=0, atol=2e-4)
+    du = np.linalg.norm(a[:, 3:6] - b[:, 3:6], axis=1) / np.maximum(np.linalg.norm(b[:, 3:6], axis=1), 1e-3)
+    assert np.median(du) < 1e-5 and np.quantile(du, 0.99) < 1e-3, (np.median(du), np.quantile(du, 0.99))
+    for l, d in enumerate(out["diag"]):
+        assert d["energy_f"] == pytest.approx(fx["diag"][l][6], rel=1e-5)
+        assert d["n_e"] == pytest.approx(fx["diag"][l][3], rel=5e-3, abs=2)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_shift_is_an_exact_copy(precision):
+    """qk_moving_window on 96 x 64 x 48 cells: E, B, Be move one plane toward
+    -x bit-exactly, the plane nx - 1 is zero, J does not move, particles of
+    the plane i = 0 are erased and the others keep offsets and momenta."""
+    nx, ny, nz = 96, 64, 48
+    dt = 0.2
+    rng = np.random.default_rng(5)
+    E = rng.uniform(-1, 1, (nx, ny, nz, 3))
+    B = rng.uniform(-1, 1, (nx, ny, nz, 3))
+    n = 200_000
+    rec = np.zeros((n, 8))
+    rec[:, 0] = rng.uniform(1, nx - 2, n)
+    rec[:, 1] = rng.uniform(1, ny - 2, n)
+    rec[:, 2] = rng.uniform(1, nz - 2, n)
+    rec[:, 3:6] = rng.normal(0, 0.1, (n, 3))
+    rec[:, 6] = -0.01
+    rec[:, 7] = -1
+    ids = np.arange(n, dtype=np.int32)
+    print(f"shift {precision} {nx}x{ny}x{nz}, {n} particles", flush=True)
+    with qa.Region(nx, ny, nz, 0.5, 0.5, 0.5, dt, precision=precision, track_ids=True) as r:
+        r.set_fields(E, B)
+        r.add_records(rec, ids)
+        r.init_boundaries()
+        r.step(1)
+        E0, B0, J0, Be0 = r.get_fields()
+        p0, i0 = all_particles(r)
+        r.moving_window()
+        E1, B1, J1, Be1 = r.get_fields()
+        p1, i1 = all_particles(r)
+    for a0, a1 in ((E0, E1), (B0, B1), (Be0, Be1)):
+        np.testing.assert_array_equal(a1[:-1], a0[1:])
+        assert not a1[-1].any()
+    np.testing.assert_array_equal(J1, J0)
+    keep = p0[:, 0] >= 1.0
+    np.testing.assert_array_equal(i1, i0[keep])
+    np.testing.assert_array_equal(p1[:, 0], p0[keep, 0] - 1)
+    np.testing.assert_array_equal(p1[:, 1:], p0[keep, 1:])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_max_w(precision):
+    nx, ny, nz = 64, 40, 36
+    rng = np.random.default_rng(9)
+    E = rng.normal(0, 1, (nx, ny, nz, 3))
+    B = rng.normal(0, 1, (nx, ny, nz, 3))
+    E[50, 7, 9, 1] = 12.0
+    print(f"max_w {precision}", flush=True)
+    with qa.Region(nx, ny, nz, 0.5, 0.5, 0.5, 0.2, precision=precision) as r:
+        r.set_fields(E, B)
+        Eg, Bg, _, _ = r.get_fields()   # the stored (rounded) values
+        w = (Eg ** 2).sum(-1) + (Bg ** 2).sum(-1)
+        for lo, hi in ((0, nx), (nx - 6, nx - 4), (50, 51), (10, 10)):
+            got = r.max_w(lo, hi)
+            want = w[lo:hi].max() if hi > lo else 0.0
+            assert got == pytest.approx(want, rel=1e-14), (lo, hi)
+        with pytest.raises(qa.QuillError):
+            r.max_w(-1, 3)
+
+
+def test_tiled_tiles_follow_the_window():
+    """FP32 tiled deposit with shifts and no re-sort between them (no fills,
+    sort_every = 0): the tile windows follow the shifted particles, so the
+    result equals the global deposit up to the order of float sums and the
+    pushes stay in their windows."""
+    fx = gio.load("mw_on")
+    c = dict(mw_case.inputs("on"), density=0.0)
+    out_t = run_mw(fx, "on", "fp32", "tiled", c=c, sort_every=0)
+    out_g = run_mw(fx, "on", "fp32", "global", c=c, sort_every=0)
+    assert out_t["trace"][-1, 0] >= 6
+    for key in ("E", "B", "J", "Be"):
+        err = rel_l2(out_t[key], out_g[key])
+        assert err < 1e-5, f"{key}: rel L2 {err:.3e}"
+    np.testing.assert_array_equal(out_t["ids"], out_g["ids"])
+    np.testing.assert_allclose(out_t["part"][:, :6], out_g["part"][:, :6], rtol=1e-4, atol=1e-5)
+    assert out_t["fallback"] < 0.02

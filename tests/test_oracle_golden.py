@@ -68,6 +68,42 @@ def test_run_matches_reference(name):
         np.testing.assert_allclose(mine, ref[l], rtol=1e-11, atol=0, err_msg=f"{name} step {l}")
 
 
+@pytest.mark.parametrize("mode", ["on", "auto"])
+def test_moving_window_matches_reference(mode):
+    """moving_window (src/advance.cpp:389-454), get_max_w
+    (src/low-level.cpp:512-532) and fill_cell_by_particles (src/init.cpp:402-478)
+    under main()'s window logic, against the reference harness."""
+    import mw_case
+    fx = gio.load(f"mw_{mode}")
+    h = hdr(fx)
+    c = mw_case.inputs(mode)
+    o = make_oracle(h)
+    o.field(oracle.E)[:] = fx["E_in"]
+    o.field(oracle.B)[:] = fx["B_in"]
+    o.add_particles(fx["part_in"], fx["ids_in"])
+    o.init_boundaries()
+    diag, trace = mw_case.drive(o, c, lambda cells, dens, npic, first: o.fill_cells(-1.0, cells, dens, npic, 0.0,
+                                                                                   first))
+    np.testing.assert_array_equal(trace[:, 0], fx["trace"][:, 0])
+    np.testing.assert_allclose(trace[:, 1], fx["trace"][:, 1], rtol=1e-12)
+    assert fx["trace"][-1, 0] >= 4   # the window moved
+    for key, which in (("E", oracle.E), ("B", oracle.B), ("J", oracle.J), ("Be", oracle.BE)):
+        err = rel_err(o.field(which), fx[key])
+        assert err < 1e-11, f"mw_{mode} {key}: rel err {err:.3e}"
+    rec, ids = o.particles()
+    order = np.argsort(ids, kind="stable")
+    rec, ids = rec[order], ids[order]
+    np.testing.assert_array_equal(ids, fx["ids"])
+    assert (ids >= mw_case.ID_BASE).sum() > 100   # column fills survive
+    np.testing.assert_allclose(rec[:, :3], fx["part"][:, :3], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(rec[:, 3:6], fx["part"][:, 3:6], rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(rec[:, 7], fx["part"][:, 7], rtol=1e-14)
+    for l, d in enumerate(diag):
+        mine = [d.N_e, d.N_p, d.N_ph, d.n_e, d.n_p, d.n_ph, d.energy_f, d.energy_e, d.energy_p,
+                d.energy_ph, d.energy_ion, d.n_ion]
+        np.testing.assert_allclose(mine, fx["diag"][l], rtol=1e-11, atol=0, err_msg=f"mw_{mode} step {l}")
+
+
 def test_laser_matches_reference():
     fx = gio.load("laser_cos")
     for hk, ek, bk in (("header", "E", "B"), ("header2", "E2", "B2")):
@@ -146,3 +182,27 @@ def test_oracle_qed_statistics_match_reference():
     assert ref[:, 0].mean() > 10000 and ref[:, 1].mean() > 10   # the case exercises both processes
     bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES, z) if not abs(v) < 4]
     assert not bad, f"oracle QED statistics differ from the reference: {bad}"
+
+
+@pytest.mark.parametrize("mode", ["ti", "nl"])
+def test_oracle_merging_matches_reference(mode):
+    """pmerging (src/mc.cpp:202-462) of the oracle against 16 merges of the
+    reference with its own random stream: surviving counts and sum q^2 per
+    species agree as distributions (|z| < 4); each species' total charge is
+    kept to round-off, as the final rescale makes it."""
+    import merge_case
+    c = merge_case.inputs()
+    ref = gio.load("merge_stats")[mode]
+    mine = []
+    for seed in range(1, merge_case.SEEDS + 1):
+        o = oracle.Oracle(c["nx"], c["ny"], c["nz"], 0.3, 0.3, 0.3, 0.1, seed=seed)
+        o.add_particles(c["rec"], c["ids"])
+        o.pmerging(mode, merge_case.PPD[mode], [merge_case.ION_CMR])
+        rec, _ = o.particles()
+        mine.append(merge_case.summary(rec))
+    mine = np.array(mine)
+    z = merge_case.zscores(mine, ref)
+    bad = [(n, round(float(v), 2)) for n, v in zip(merge_case.NAMES, z) if not n.endswith("_q") and not abs(v) < 4]
+    assert not bad, f"oracle merging differs from the reference: {bad}"
+    np.testing.assert_allclose(mine[:, 1::3], ref[:, 1::3], rtol=1e-12)
+    assert (ref[:, 0] < len(c["rec"])).all()
