@@ -32,7 +32,8 @@ SPECIES_NAMES = {"electron": ELECTRON, "positron": POSITRON, "photon": PHOTON}
 EXPORTS = ["qk_create", "qk_destroy", "qk_last_error", "qk_set_fields", "qk_get_fields", "qk_add_particles",
            "qk_load_uniform_plasma", "qk_init_laser_cos", "qk_init_boundaries", "qk_step", "qk_sync",
            "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_last_fallback",
-           "qk_get_stream", "qk_pmerging", "qk_moving_window", "qk_max_w", "qk_fill_cells"]
+           "qk_get_stream", "qk_pmerging", "qk_moving_window", "qk_max_w", "qk_fill_cells", "qk_tape_begin",
+           "qk_adjoint"]
 
 
 class QuillError(RuntimeError):
@@ -101,6 +102,8 @@ def load_library(path=LIB_PATH):
     L.qk_moving_window.argtypes = [vp]
     L.qk_max_w.argtypes = [vp, C.c_int, C.c_int, dp]
     L.qk_fill_cells.argtypes = [vp, C.c_int, C.c_int64, ip, dp, ip, C.c_double, C.c_int32, i64p]
+    L.qk_tape_begin.argtypes = [vp, C.c_int]
+    L.qk_adjoint.argtypes = [vp, dp, dp, C.POINTER(dp), dp, dp, C.POINTER(dp)]
     for name in EXPORTS:
         if name not in ("qk_last_error",):
             getattr(L, name).restype = C.c_int
@@ -290,6 +293,41 @@ class Region:
         _check(load_library().qk_fill_cells(self._h, species, len(cells), _ip(cells), _dp(density), _ip(npic),
                                             ux0, first_id, C.byref(added)), "qk_fill_cells")
         return added.value
+
+    # -- adjoint ------------------------------------------------------------
+    def tape_begin(self, every):
+        """Record the following steps for adjoint(): a snapshot now and every `every` steps."""
+        _check(load_library().qk_tape_begin(self._h, every), "qk_tape_begin")
+        self._tape_n = {s: self.count(s) for s in range(ION0 + self.params.n_ion)}
+
+    def adjoint(self, lam_E=None, lam_B=None, lam_parts=None):
+        """Adjoint of the recorded steps.  lam_E, lam_B: (nx, ny, nz, 3) adjoints
+        of the final fields; lam_parts: {species: (n, 7)} adjoints of the final
+        x y z ux uy uz q of the n slots the species had at tape_begin (in the
+        order added; zero for deleted particles).  Returns (grad_E0, grad_B0,
+        {species: (n, 7)}) at tape_begin."""
+        L = load_library()
+        shape = self.shape + (3,)
+        lE = np.ascontiguousarray(lam_E, np.float64) if lam_E is not None else None
+        lB = np.ascontiguousarray(lam_B, np.float64) if lam_B is not None else None
+        gE, gB = np.zeros(shape), np.zeros(shape)
+        lp = (C.POINTER(C.c_double) * MAX_SPECIES)()
+        gp = (C.POINTER(C.c_double) * MAX_SPECIES)()
+        keep, grads = [], {}
+        for s, n in self._tape_n.items():
+            if n == 0:
+                continue
+            if lam_parts is not None and s in lam_parts:
+                a = np.ascontiguousarray(lam_parts[s], np.float64)
+                if a.shape != (n, 7):
+                    raise ValueError(f"species {s}: adjoint shape {a.shape} != {(n, 7)}")
+                keep.append(a)
+                lp[s] = _dp(a)
+            grads[s] = np.zeros((n, 7))
+            gp[s] = _dp(grads[s])
+        _check(L.qk_adjoint(self._h, _dp(lE) if lE is not None else None, _dp(lB) if lB is not None else None, lp,
+                            _dp(gE), _dp(gB), gp), "qk_adjoint")
+        return gE, gB, grads
 
     # -- stepping -----------------------------------------------------------
     def step(self, nsteps=1, diag=False):

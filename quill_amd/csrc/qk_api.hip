@@ -63,6 +63,27 @@ struct SpeciesBuf {
     int ishift = 0;         // moving-window shifts since the last sort
 };
 
+// The recorded trajectory of qk_tape_begin: snapshots of the whole device
+// state (the 15 field arrays and each species' record arrays) at steps 0,
+// every, 2 every, ...  The adjoint recomputes one segment at a time from its
+// snapshot, keeping that segment's per-step states, and runs the step
+// adjoints backwards over them.
+struct Tape {
+    bool active = false;
+    bool replay = false;          // recomputing a segment: no snapshots
+    bool dirty = false;           // the state was changed outside qk_step
+    int every = 0;
+    int steps = 0;                // steps since qk_tape_begin
+    int64_t n[QK_MAX_SPECIES] = {};
+    size_t field_bytes = 0, bytes = 0;
+    std::vector<void *> ckpt;     // snapshots at steps 0, every, ...
+    std::vector<int> ckpt_jcur;
+    void *seg = nullptr;          // every per-step states of one segment
+    void *fin = nullptr;          // the end state, restored after the adjoint
+    double *lam = nullptr;        // 12 field adjoints: E, B, Be, J
+    double *lpart = nullptr;      // 7 record adjoints per species
+};
+
 }  // namespace
 
 struct qk_ctx {
@@ -102,6 +123,7 @@ struct qk_ctx {
     unsigned long long *d_maxw = nullptr, *h_maxw = nullptr;
     void *d_fill = nullptr;
     size_t fill_cap = 0;
+    Tape tape;
 };
 
 namespace {
@@ -480,6 +502,234 @@ int qed_particle_phase(qk_ctx *c, const Fields<real> &f, const Current<real> &jc
     return QK_OK;
 }
 
+// ---- adjoint tape -----------------------------------------------------------
+
+// the record arrays a snapshot keeps per species: cell, ox oy oz ux uy uz q
+size_t species_state_bytes(const qk_ctx *c, int64_t n) { return (size_t)n * (4 + 7 * c->rsize); }
+
+int tape_save(qk_ctx *c, void *dst)
+{
+    char *d = (char *)dst;
+    HIP_TRY(hipMemcpyAsync(d, c->fmem, c->tape.field_bytes, hipMemcpyDeviceToDevice, c->stream));
+    d += c->tape.field_bytes;
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        const int64_t n = c->tape.n[s];
+        if (n == 0) continue;
+        const SoA &b = c->sp[s].buf[c->sp[s].cur];
+        HIP_TRY(hipMemcpyAsync(d, b.cell, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+        d += (size_t)n * 4;
+        for (int a = 0; a < 7; ++a) {
+            HIP_TRY(hipMemcpyAsync(d, b.a[a], (size_t)n * c->rsize, hipMemcpyDeviceToDevice, c->stream));
+            d += (size_t)n * c->rsize;
+        }
+    }
+    return QK_OK;
+}
+
+int tape_load(qk_ctx *c, const void *src)
+{
+    const char *d = (const char *)src;
+    HIP_TRY(hipMemcpyAsync(c->fmem, d, c->tape.field_bytes, hipMemcpyDeviceToDevice, c->stream));
+    d += c->tape.field_bytes;
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        const int64_t n = c->tape.n[s];
+        if (n == 0) continue;
+        const SoA &b = c->sp[s].buf[c->sp[s].cur];
+        HIP_TRY(hipMemcpyAsync(b.cell, d, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+        d += (size_t)n * 4;
+        for (int a = 0; a < 7; ++a) {
+            HIP_TRY(hipMemcpyAsync(b.a[a], d, (size_t)n * c->rsize, hipMemcpyDeviceToDevice, c->stream));
+            d += (size_t)n * c->rsize;
+        }
+    }
+    return QK_OK;
+}
+
+// views of a saved state: the fields (E, Be) and one species' records
+template <typename real>
+Fields<real> saved_fields(const qk_ctx *c, const void *state)
+{
+    const size_t fb = c->tape.field_bytes / 15;
+    const char *m = (const char *)state;
+    Fields<real> f;
+    real *F[9];
+    for (int a = 0; a < 9; ++a) F[a] = (real *)(m + a * fb);
+    f.ex = F[0]; f.ey = F[1]; f.ez = F[2];
+    f.bx = F[3]; f.by = F[4]; f.bz = F[5];
+    f.bex = F[6]; f.bey = F[7]; f.bez = F[8];
+    return f;
+}
+
+template <typename real>
+Species<real> saved_species(const qk_ctx *c, const void *state, int s)
+{
+    const char *d = (const char *)state + c->tape.field_bytes;
+    for (int t = 0; t < s; ++t) d += species_state_bytes(c, c->tape.n[t]);
+    const int64_t n = c->tape.n[s];
+    Species<real> v;
+    v.cell = (int *)d;
+    d += (size_t)n * 4;
+    real *a[7];
+    for (int k = 0; k < 7; ++k) {
+        a[k] = (real *)d;
+        d += (size_t)n * c->rsize;
+    }
+    v.ox = a[0]; v.oy = a[1]; v.oz = a[2];
+    v.ux = a[3]; v.uy = a[4]; v.uz = a[5];
+    v.q = a[6];
+    v.chi = nullptr;
+    v.id = nullptr;
+    return v;
+}
+
+int tape_snapshot(qk_ctx *c)
+{
+    void *m = nullptr;
+    HIP_TRY(hipMalloc(&m, c->tape.bytes));
+    c->tape.ckpt.push_back(m);
+    c->tape.ckpt_jcur.push_back(c->jcur);
+    return tape_save(c, m);
+}
+
+void tape_free(qk_ctx *c)
+{
+    Tape &t = c->tape;
+    for (void *m : t.ckpt) (void)hipFree(m);
+    t.ckpt.clear();
+    t.ckpt_jcur.clear();
+    if (t.seg) (void)hipFree(t.seg);
+    if (t.fin) (void)hipFree(t.fin);
+    if (t.lam) (void)hipFree(t.lam);
+    if (t.lpart) (void)hipFree(t.lpart);
+    t.seg = t.fin = nullptr;
+    t.lam = t.lpart = nullptr;
+    t.active = t.replay = t.dirty = false;
+    t.steps = 0;
+}
+
+template <typename real>
+int step_impl(qk_ctx *c, int nsteps, qk_diag *diag);
+
+// the adjoint sweep of qk_adjoint (see there)
+template <typename real>
+int adjoint_impl(qk_ctx *c, const double *lam_E, const double *lam_B, const double *const *lam_part, double *grad_E,
+                 double *grad_B, double *const *grad_part)
+{
+    Tape &t = c->tape;
+    const int64_t nc = c->g.ncell;
+    const int T = t.steps;
+    // adjoint buffers
+    int64_t np_total = 0;
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) np_total += t.n[s];
+    if (!t.lam) HIP_TRY(hipMalloc(&t.lam, (size_t)nc * 12 * sizeof(double)));
+    if (!t.lpart && np_total > 0) HIP_TRY(hipMalloc(&t.lpart, (size_t)np_total * 7 * sizeof(double)));
+    if (!t.fin) HIP_TRY(hipMalloc(&t.fin, t.bytes));
+    if (!t.seg) HIP_TRY(hipMalloc(&t.seg, t.bytes * (size_t)t.every));
+    double *lam[12];
+    for (int a = 0; a < 12; ++a) lam[a] = t.lam + (size_t)a * nc;
+    double *lpart[QK_MAX_SPECIES][7] = {};
+    {
+        double *m = t.lpart;
+        for (int s = 0; s < QK_MAX_SPECIES; ++s)
+            for (int a = 0; a < 7; ++a) {
+                lpart[s][a] = m;
+                m += t.n[s];
+            }
+    }
+    // the end state, and its bookkeeping, to come back to
+    const int jcur_end = c->jcur;
+    const uint32_t step_end = c->step_index;
+    QK_TRY(tape_save(c, t.fin));
+    // seeds: fields AoS -> SoA, records AoS (x y z ux uy uz q) -> SoA
+    {
+        std::vector<double> h((size_t)nc * 12, 0.0);
+        for (int64_t m = 0; m < nc; ++m)
+            for (int a = 0; a < 3; ++a) {
+                if (lam_E) h[(size_t)a * nc + m] = lam_E[3 * m + a];
+                if (lam_B) h[(size_t)(3 + a) * nc + m] = lam_B[3 * m + a];
+            }
+        HIP_TRY(hipMemcpyAsync(t.lam, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            const int64_t n = t.n[s];
+            if (n == 0) continue;
+            std::vector<double> hp((size_t)n * 7, 0.0);
+            if (lam_part && lam_part[s])
+                for (int64_t p = 0; p < n; ++p)
+                    for (int a = 0; a < 7; ++a) hp[(size_t)a * n + p] = lam_part[s][7 * p + a];
+            HIP_TRY(hipMemcpyAsync(lpart[s][0], hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice,
+                                   c->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    // segments, last first
+    t.replay = true;
+    const int nseg = (T + t.every - 1) / t.every;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int first = seg * t.every, last = std::min(T, first + t.every);
+        QK_TRY(tape_load(c, t.ckpt[seg]));
+        c->jcur = t.ckpt_jcur[seg];
+        for (int n = first; n < last; ++n) {
+            QK_TRY(tape_save(c, (char *)t.seg + (size_t)(n - first) * t.bytes));
+            if (n + 1 < last) QK_TRY(step_impl<real>(c, 1, nullptr));
+        }
+        for (int n = last - 1; n >= first; --n) {
+            const void *st = (const char *)t.seg + (size_t)(n - first) * t.bytes;
+            HIP_TRY(launch_adj_fields(c->stream, c->p.solver, c->g, c->coef, lam));
+            const Fields<real> f = saved_fields<real>(c, st);
+            for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+                if (t.n[s] == 0) continue;
+                PushArgs a{};
+                a.pusher = c->p.pusher;
+                a.kind = 0;
+                a.cmr = c->sp[s].cmr;
+                a.external_bz = c->p.external_bz;
+                HIP_TRY(launch_adj_particles<real>(c->stream, c->g, f, saved_species<real>(c, st, s), t.n[s], a, lam,
+                                                   lpart[s]));
+            }
+        }
+    }
+    t.replay = false;
+    // Be of step 0 came from interpolate_be, and init_boundaries zeroed the
+    // boundary planes of the fields set before it: their adjoints are zero
+    HIP_TRY(launch_adj_interp_be(c->stream, c->g, lam));
+    {
+        Fields<double> lf;
+        lf.ex = lam[0]; lf.ey = lam[1]; lf.ez = lam[2];
+        lf.bx = lam[3]; lf.by = lam[4]; lf.bz = lam[5];
+        lf.bex = lam[6]; lf.bey = lam[7]; lf.bez = lam[8];
+        HIP_TRY(launch_init_boundaries<double>(c->stream, c->p.solver, c->g, lf));
+    }
+    // results
+    {
+        std::vector<double> h((size_t)nc * 6);
+        HIP_TRY(hipMemcpyAsync(h.data(), t.lam, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        std::vector<std::vector<double>> hp(QK_MAX_SPECIES);
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            if (t.n[s] == 0 || !grad_part || !grad_part[s]) continue;
+            hp[s].resize((size_t)t.n[s] * 7);
+            HIP_TRY(hipMemcpyAsync(hp[s].data(), lpart[s][0], hp[s].size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int64_t m = 0; m < nc; ++m)
+            for (int a = 0; a < 3; ++a) {
+                if (grad_E) grad_E[3 * m + a] = h[(size_t)a * nc + m];
+                if (grad_B) grad_B[3 * m + a] = h[(size_t)(3 + a) * nc + m];
+            }
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            if (hp[s].empty()) continue;
+            const int64_t n = t.n[s];
+            for (int64_t p = 0; p < n; ++p)
+                for (int a = 0; a < 7; ++a) grad_part[s][7 * p + a] = hp[s][(size_t)a * n + p];
+        }
+    }
+    QK_TRY(tape_load(c, t.fin));
+    c->jcur = jcur_end;
+    c->step_index = step_end;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
 template <typename real>
 int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
 {
@@ -497,6 +747,11 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
     // the QED kernels deposit through global atomics
     const bool tiled = c->p.deposit == QK_DEPOSIT_TILED && !qed;
     for (int l = 0; l < nsteps; ++l) {
+        Tape &tp = c->tape;
+        if (tp.active && !tp.replay) {
+            if (tp.steps > 0 && tp.steps % tp.every == 0) QK_TRY(tape_snapshot(c));
+            ++tp.steps;
+        }
         // cell-tile sort: the cadence, or an unsorted tail (appended particles,
         // pushed by the global kernel meanwhile) above 1/16 of the sorted part
         // (photons only with QED, where pair creation leaves free slots)
@@ -767,6 +1022,11 @@ int qk_destroy(qk_ctx *c)
     if (c->h_count) note(hipHostFree(c->h_count));
     if (c->d_merge) note(hipFree(c->d_merge));
     if (c->h_merge) note(hipHostFree(c->h_merge));
+    for (void *m : c->tape.ckpt) note(hipFree(m));
+    if (c->tape.seg) note(hipFree(c->tape.seg));
+    if (c->tape.fin) note(hipFree(c->tape.fin));
+    if (c->tape.lam) note(hipFree(c->tape.lam));
+    if (c->tape.lpart) note(hipFree(c->tape.lpart));
     if (c->d_maxw) note(hipFree(c->d_maxw));
     if (c->h_maxw) note(hipHostFree(c->h_maxw));
     if (c->d_fill) note(hipFree(c->d_fill));
@@ -780,6 +1040,7 @@ int qk_destroy(qk_ctx *c)
 int qk_set_fields(qk_ctx *c, const double *E, const double *B)
 {
     if (!c) return fail(QK_ERR_INVALID, "null context");
+    c->tape.dirty = c->tape.active;
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (E) QK_TRY(upload3(c, E, c->F[0], c->F[1], c->F[2]));
     if (B) QK_TRY(upload3(c, B, c->F[3], c->F[4], c->F[5]));
@@ -806,6 +1067,7 @@ int qk_add_particles(qk_ctx *c, int s, int64_t n, const double *x, const double 
     if (!c) return fail(QK_ERR_INVALID, "null context");
     if (!species_valid(c, s)) return fail(QK_ERR_INVALID, "invalid species");
     if (n < 0 || (n > 0 && (!x || !y || !z || !ux || !uy || !uz || !q))) return fail(QK_ERR_INVALID, "bad particle arrays");
+    c->tape.dirty = c->tape.active;
     // place(): keep particles in is_inside cells (src/plist.cpp:22-31)
     std::vector<int64_t> keep;
     keep.reserve((size_t)n);
@@ -855,6 +1117,7 @@ int qk_load_uniform_plasma(qk_ctx *c, int s, int ppc, double density, double sig
                            int64_t *added)
 {
     if (!c) return fail(QK_ERR_INVALID, "null context");
+    c->tape.dirty = c->tape.active;
     if (!species_valid(c, s) || s == QK_PHOTON) return fail(QK_ERR_INVALID, "invalid species for a plasma");
     if (ppc <= 0) return fail(QK_ERR_INVALID, "ppc must be positive");
     if (ilo < 1) ilo = 1;
@@ -878,6 +1141,7 @@ int qk_load_uniform_plasma(qk_ctx *c, int s, int ppc, double density, double sig
 int qk_init_laser_cos(qk_ctx *c, const double L[16])
 {
     if (!c || !L) return fail(QK_ERR_INVALID, "null argument");
+    c->tape.dirty = c->tape.active;
     hipError_t e = c->rsize == 4 ? launch_laser_cos<float>(c->stream, c->g, fields_of<float>(c), L)
                                  : launch_laser_cos<double>(c->stream, c->g, fields_of<double>(c), L);
     if (e == hipErrorInvalidValue) return fail(QK_ERR_INVALID, "laser direction undefined (x1 = y0 = z0 = 0)");
@@ -889,6 +1153,7 @@ int qk_init_laser_cos(qk_ctx *c, const double L[16])
 int qk_init_boundaries(qk_ctx *c)
 {
     if (!c) return fail(QK_ERR_INVALID, "null context");
+    c->tape.dirty = c->tape.active;
     if (c->rsize == 4) {
         HIP_TRY(launch_init_boundaries<float>(c->stream, c->p.solver, c->g, fields_of<float>(c)));
         HIP_TRY(launch_interpolate_be<float>(c->stream, c->g, fields_of<float>(c)));
@@ -1001,6 +1266,7 @@ int qk_compute_rho(qk_ctx *c, double *rho)
 int qk_pmerging(qk_ctx *c, int mode, const double *ppd)
 {
     if (!c || !ppd) return fail(QK_ERR_INVALID, "null argument");
+    c->tape.dirty = c->tape.active;
     if (mode != 1 && mode != 2) return fail(QK_ERR_INVALID, "merging mode must be 1 (ti) or 2 (nl)");
     for (int s = 0; s < QK_ION0 + c->p.n_ion; ++s)
         if (!(ppd[s] >= 0 && ppd[s] < 1)) return fail(QK_ERR_INVALID, "ppd must lie in [0, 1)");
@@ -1010,6 +1276,7 @@ int qk_pmerging(qk_ctx *c, int mode, const double *ppd)
 int qk_moving_window(qk_ctx *c)
 {
     if (!c) return fail(QK_ERR_INVALID, "null context");
+    c->tape.dirty = c->tape.active;
     if (c->rsize == 4) HIP_TRY(launch_shift_fields<float>(c->stream, c->g, fields_of<float>(c)));
     else HIP_TRY(launch_shift_fields<double>(c->stream, c->g, fields_of<double>(c)));
     for (int s = 0; s < QK_MAX_SPECIES; ++s) {
@@ -1043,6 +1310,7 @@ int qk_fill_cells(qk_ctx *c, int s, int64_t ncells, const int32_t *cells, const 
                   const int32_t npic[3], double ux0, int32_t first_id, int64_t *added)
 {
     if (!c) return fail(QK_ERR_INVALID, "null context");
+    c->tape.dirty = c->tape.active;
     if (!species_valid(c, s)) return fail(QK_ERR_INVALID, "invalid species");
     if (ncells < 0 || (ncells > 0 && (!cells || !density)) || !npic) return fail(QK_ERR_INVALID, "bad cell list");
     if (npic[0] <= 0 || npic[1] <= 0 || npic[2] <= 0 || (int64_t)npic[0] * npic[1] * npic[2] > 4096)
@@ -1088,6 +1356,39 @@ int qk_fill_cells(qk_ctx *c, int s, int64_t ncells, const int32_t *cells, const 
     b.n += total;
     if (added) *added = total;
     return QK_OK;
+}
+
+int qk_tape_begin(qk_ctx *c, int every)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (every <= 0) return fail(QK_ERR_INVALID, "checkpoint interval must be positive");
+    const qk_params &p = c->p;
+    if (p.qed || p.merging || p.deposit != QK_DEPOSIT_GLOBAL || p.sort_every != 0)
+        return fail(QK_ERR_INVALID, "the adjoint needs qed = 0, merging = 0, deposit = QK_DEPOSIT_GLOBAL, sort_every = 0");
+    if (c->sp[QK_PHOTON].active && c->sp[QK_PHOTON].n > 0) return fail(QK_ERR_INVALID, "the adjoint covers charged species only");
+    tape_free(c);
+    Tape &t = c->tape;
+    t.every = every;
+    t.field_bytes = (((size_t)c->g.ncell * c->rsize + 255) & ~(size_t)255) * 15;
+    t.bytes = t.field_bytes;
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        t.n[s] = c->sp[s].active ? c->sp[s].n : 0;
+        t.bytes += species_state_bytes(c, t.n[s]);
+    }
+    t.active = true;
+    QK_TRY(tape_snapshot(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+int qk_adjoint(qk_ctx *c, const double *lam_E, const double *lam_B, const double *const *lam_part, double *grad_E,
+               double *grad_B, double *const *grad_part)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    if (!c->tape.active) return fail(QK_ERR_STATE, "no tape: call qk_tape_begin before the steps");
+    if (c->tape.dirty) return fail(QK_ERR_STATE, "fields or particles changed outside qk_step since qk_tape_begin");
+    return c->rsize == 4 ? adjoint_impl<float>(c, lam_E, lam_B, lam_part, grad_E, grad_B, grad_part)
+                         : adjoint_impl<double>(c, lam_E, lam_B, lam_part, grad_E, grad_B, grad_part);
 }
 
 int qk_last_timing(qk_ctx *c, double *ms_total, double *ms_particles, double *ms_fields)

@@ -125,6 +125,21 @@ template <typename real>
 hipError_t launch_fill_cells(hipStream_t st, const Species<real> &s, int64_t first, int64_t ncells, const int *cells,
                              const double *density, const int *idbase, const int npic[3], double sign, double ux0);
 
+// ---- adjoint of the deterministic step --------------------------------------
+
+// the transposes of fadvance on the FP64 adjoint arrays lam = lE[3], lB[3],
+// lBe[3], lJ[3]: interpolate_be, B half step, E step (which sets lJ), B half
+// step, in that order
+hipError_t launch_adj_fields(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, double *const lam[12]);
+// the transpose of interpolate_be alone (for Be of the first step)
+hipError_t launch_adj_interp_be(hipStream_t st, const Grid &g, double *const lam[12]);
+// the adjoint of one species' push from its recorded state s and fields f
+// (E, Be) of the step: lpart = lx[3], lu[3], lq (FP64, per slot) go from after
+// the step to before it; lE and lBe receive the gather's node adjoints
+template <typename real>
+hipError_t launch_adj_particles(hipStream_t st, const Grid &g, const Fields<real> &f, const Species<real> &s,
+                                int64_t n, const PushArgs &a, double *const lam[12], double *const lpart[7]);
+
 // the field-energy sum is spread over this many FP64 slots (one atomic per
 // workgroup lands on slot blockIdx mod kEnergySlots) and summed on the host
 constexpr int kEnergySlots = 256;
