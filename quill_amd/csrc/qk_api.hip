@@ -61,6 +61,7 @@ struct SpeciesBuf {
     int nchunks = 0;        // host copy after the last sort
     int steps_since_sort = 0;
     int ishift = 0;         // moving-window shifts since the last sort
+    int *tile_start = nullptr;   // ntiles + 1: the tiles of slots [0, n_sorted) (the next sort's source)
 };
 
 // The recorded trajectory of qk_tape_begin: snapshots of the whole device
@@ -99,7 +100,7 @@ struct qk_ctx {
     bool have_step = false;
     SpeciesBuf sp[QK_MAX_SPECIES];
     // shared sort buffers (allocated at the first sort)
-    int *s_counts = nullptr, *s_offsets = nullptr, *s_tile_start = nullptr, *s_tile_mult = nullptr,
+    int *s_counts = nullptr, *s_imm = nullptr, *s_offsets = nullptr, *s_tile_start = nullptr, *s_tile_mult = nullptr,
         *s_chunk_n = nullptr, *s_chunk_off = nullptr, *s_scratch = nullptr;
     int *h_sortinfo = nullptr;      // pinned: [live count, chunk count]
     double *d_diag = nullptr;       // [QK_MAX_SPECIES][kDiagPerSpecies] + field energy
@@ -251,8 +252,12 @@ int ensure_species(qk_ctx *c, int s, int64_t extra)
     }
     const int64_t need = b.n + extra;
     if (need <= b.cap) return QK_OK;
-    if (need >= ((int64_t)1 << 31)) return fail(QK_ERR_CAPACITY, "more than 2^31 particles in one species");
-    int64_t cap = c->p.capacity > 0 ? c->p.capacity : std::max<int64_t>(need + need / 4, 1024);
+    // the push kernels address records with 32-bit byte offsets
+    const int64_t limit = ((int64_t)1 << 32) / (int64_t)c->rsize;
+    if (need >= limit)
+        return fail(QK_ERR_CAPACITY, "more than " + std::to_string(limit - 1) + " particles in one species");
+    int64_t cap = c->p.capacity > 0 ? c->p.capacity : std::min(std::max<int64_t>(need + need / 4, 1024), limit - 1);
+    if (cap >= limit) return fail(QK_ERR_CAPACITY, "capacity " + std::to_string(cap) + " exceeds the 32-bit record offsets");
     if (cap < need) {
         if (c->p.capacity > 0)
             return fail(QK_ERR_CAPACITY, "particle capacity " + std::to_string(c->p.capacity) + " exceeded (need " +
@@ -267,6 +272,7 @@ int ensure_sort_buffers(qk_ctx *c)
     if (c->s_counts) return QK_OK;
     const size_t nk = (size_t)c->tg.nkeys + 1, nt = (size_t)c->tg.ntiles + 1;
     HIP_TRY(hipMalloc(&c->s_counts, nk * sizeof(int)));
+    HIP_TRY(hipMalloc(&c->s_imm, nk * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_offsets, nk * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_tile_start, nt * sizeof(int)));
     HIP_TRY(hipMalloc(&c->s_tile_mult, nt * sizeof(int)));
@@ -283,8 +289,12 @@ int sort_species(qk_ctx *c, int s)
 {
     QK_TRY(ensure_sort_buffers(c));
     SpeciesBuf &b = c->sp[s];
+    if (!b.tile_start) HIP_TRY(hipMalloc(&b.tile_start, ((size_t)c->tg.ntiles + 1) * sizeof(int)));
     SortBuffers sb;
     sb.counts = c->s_counts;
+    sb.imm = c->s_imm;
+    sb.old_start = b.n_sorted > 0 ? b.tile_start : nullptr;
+    sb.n_sorted = b.n_sorted;
     sb.offsets = c->s_offsets;
     sb.tile_start = c->s_tile_start;
     sb.tile_mult = c->s_tile_mult;
@@ -295,6 +305,8 @@ int sort_species(qk_ctx *c, int s)
     sb.max_chunks = b.max_chunks;
     HIP_TRY(launch_sort<real>(c->stream, c->tg, view<real>(b.buf[b.cur]), view<real>(b.buf[b.cur ^ 1]), b.n, sb));
     HIP_TRY(hipMemcpyAsync(b.d_nchunks, c->s_chunk_off + c->tg.ntiles, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(b.tile_start, c->s_tile_start, ((size_t)c->tg.ntiles + 1) * sizeof(int),
+                           hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->h_sortinfo, c->s_tile_start + c->tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(c->h_sortinfo + 1, c->s_chunk_off + c->tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -946,7 +958,7 @@ int qk_create(const qk_params *params, qk_ctx **out)
     c->tg.nty = (p.ny + kTy - 1) / kTy;
     c->tg.ntz = (p.nz + kTz - 1) / kTz;
     c->tg.ntiles = c->tg.ntx * c->tg.nty * c->tg.ntz;
-    c->tg.nkeys = c->tg.ntiles;
+    c->tg.nkeys = c->tg.ntiles * kTileCells;
     c->tg.ishift = 0;
     c->rsize = p.precision == QK_FP64 ? 8 : 4;
     // solver coefficients (src/maxwell.cpp:24-33, 238-249)
@@ -1009,8 +1021,9 @@ int qk_destroy(qk_ctx *c)
             if (k.mem) note(hipFree(k.mem));
         if (b.chunks) note(hipFree(b.chunks));
         if (b.d_nchunks) note(hipFree(b.d_nchunks));
+        if (b.tile_start) note(hipFree(b.tile_start));
     }
-    int *sb[7] = {c->s_counts, c->s_offsets, c->s_tile_start, c->s_tile_mult, c->s_chunk_n, c->s_chunk_off,
+    int *sb[8] = {c->s_counts, c->s_imm, c->s_offsets, c->s_tile_start, c->s_tile_mult, c->s_chunk_n, c->s_chunk_off,
                   c->s_scratch};
     for (int *p : sb)
         if (p) note(hipFree(p));

@@ -28,8 +28,8 @@ __host__ __device__ inline int cell_j(int c) { return (c >> 10) & 1023; }
 __host__ __device__ inline int cell_k(int c) { return c & 1023; }
 
 // cell tiles of (1 << kTsx) x (1 << kTsy) x (1 << kTsz) cells, numbered
-// i-major; particles are sorted by tile (the order inside a tile is the
-// scatter's spread order, see qk_sort.hip)
+// i-major; particles are sorted by (tile, cell inside the tile) and stored
+// inside a tile in the scatter's spread order (see qk_sort.hip)
 #ifndef QK_TILE_SHIFTS
 #define QK_TILE_SHIFTS 3, 3, 3
 #endif
@@ -41,7 +41,7 @@ constexpr int kTileCells = kTx * kTy * kTz;
 struct TileGeom {
     int ntx, nty, ntz;   // tiles per axis
     int ntiles;
-    int nkeys;           // sort keys: one per tile
+    int nkeys;           // sort keys: ntiles * kTileCells
     int ishift;          // moving-window shifts since the sort: a tile's cells now start at i = 8 ti - ishift
 };
 
@@ -50,9 +50,12 @@ __host__ __device__ inline int tile_of(const TileGeom &t, int i, int j, int k)
     return ((i >> kTsx) * t.nty + (j >> kTsy)) * t.ntz + (k >> kTsz);
 }
 
+// (tile, cell inside the tile): one key per cell, so the scatter's slot
+// counters are spread over every cell rather than one per tile
 __host__ __device__ inline int sort_key(const TileGeom &t, int c)
 {
-    return tile_of(t, c >> 20, (c >> 10) & 1023, c & 1023);
+    const int i = c >> 20, j = (c >> 10) & 1023, k = c & 1023;
+    return tile_of(t, i, j, k) * kTileCells + ((((i & (kTx - 1)) << kTsy | (j & (kTy - 1))) << kTsz) | (k & (kTz - 1)));
 }
 
 // Workgroups are dealt to the 8 XCDs of the chip round-robin by block index,

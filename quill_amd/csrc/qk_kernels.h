@@ -172,8 +172,9 @@ size_t scan_scratch_ints(size_t n);
 hipError_t launch_exclusive_scan(hipStream_t st, const int *in, int *out, int n, int *scratch);
 
 struct SortBuffers {
-    int *counts;      // nkeys + 1 (particles per tile)
-    int *offsets;     // nkeys + 1 (first slot per tile; consumed as cursors by the scatter)
+    int *counts;      // nkeys + 1: particles per key (first sort); per tile, those that stay (re-sort)
+    int *imm;         // nkeys + 1: re-sort: per tile, those that arrive, then all, then the arrivals' counters
+    int *offsets;     // nkeys + 1: first slot per key (first sort)
     int *tile_start;  // ntiles + 1
     int *tile_mult;   // ntiles: inverse spread multiplier of the tile's storage order
     int *chunk_n;     // ntiles + 1
@@ -181,12 +182,15 @@ struct SortBuffers {
     int4 *chunks;     // max_chunks
     int *scratch;     // scan scratch
     int max_chunks;
+    const int *old_start;   // ntiles + 1: the tiles of the source's slots [0, n_sorted), or null
+    int64_t n_sorted;
 };
 
-// sort the live particles of src[0, n) into dst by tile, each tile's
-// particles in its spread storage order (consecutive slots hold particles
-// of distinct cells; see k_tile_chunks); writes
-// tile_start, the chunk work list and its length (chunk_off[ntiles])
+// sort the live particles of src[0, n) into dst by tile: the first sort by
+// (tile, cell) with each tile in its spread storage order (consecutive slots
+// hold particles of distinct cells), later sorts keeping each tile's order
+// (old_start: the tiles of the previous sort); writes tile_start, the chunk
+// work list and its length (chunk_off[ntiles])
 template <typename real>
 hipError_t launch_sort(hipStream_t st, const TileGeom &tg, const Species<real> &src, const Species<real> &dst,
                        int64_t n, const SortBuffers &b);

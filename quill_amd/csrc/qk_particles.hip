@@ -176,8 +176,10 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
 {
     __shared__ double win[kJWin];
     __shared__ real stage[2 * kFStage];
-    // consecutive chunks (neighbouring tiles) run on one XCD and share its L2
-    const int vb = (int)xcd_block(blockIdx.x, gridDim.x);
+    // chunks in launch order: the round-robin deal of workgroups to the XCDs
+    // spreads neighbouring tiles over all eight, which measured faster here
+    // than grouping them on one XCD (unlike the scatter, see qk_sort.hip)
+    const int vb = (int)blockIdx.x;
     if (vb >= *nchunks) return;   // whole workgroup leaves together
     const int4 ch = chunks[vb];
     const int tile = ch.x;
@@ -201,6 +203,8 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
     int ri = 0, rj = 0, rk = 0;
     bool in_win = false;
     auto dep = [&](int, int, int, int base, real q, real pos[3], const real d[3], DiagAcc &acc_d) {
+        // one accumulator per branch: a shared one with per-lane atomics
+        // (path split once, two masked atomics per term) measured 2.5 % slower
         if (in_win) {
             LdsAcc<real> acc(win, ri, rj, rk);
             deposit_path<real>(acc, q, pos, d);
@@ -211,21 +215,19 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
         }
     };
     const int64_t end = (int64_t)ch.y + ch.z;
-#ifndef QK_PREFETCH
-#define QK_PREFETCH 1
-#endif
-    // the next record is loaded while the current one is processed
-    PRec<real> nxt;
-    int64_t p = (int64_t)ch.y + threadIdx.x;
-    if (QK_PREFETCH && p < end) load_rec(s, p, nxt);
-    for (; p < end; p += kTBlock) {
-        PRec<real> r;
-        if (QK_PREFETCH) {
-            r = nxt;
-            if (p + kTBlock < end) load_rec(s, p + kTBlock, nxt);
-        } else {
-            load_rec(s, p, r);
-        }
+    // the records of the next two rounds are in flight while one is
+    // processed; slots as 32-bit indices: the record arrays are addressed as
+    // scalar base + one shared 32-bit byte offset (capacity keeps slot * size
+    // below 2^32)
+    PRec<real> nxt, nxt2;
+    uint32_t p = (uint32_t)ch.y + threadIdx.x;
+    const uint32_t end32 = (uint32_t)end;
+    if (p < end32) load_rec32(s, p, nxt);
+    if (p + kTBlock < end32) load_rec32(s, p + kTBlock, nxt2);
+    for (; p < end32; p += kTBlock) {
+        PRec<real> r = nxt;
+        nxt = nxt2;
+        if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, nxt2);
         if (r.cell >= 0) {
             ri = cell_i(r.cell) - i0;
             rj = cell_j(r.cell) - j0;
@@ -237,7 +239,7 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
             if (in_win) load_corners_stage(stage, ri, rj, rk, nodes);
             else load_corners(f, cell_base(g, r.cell), g.sx, g.sy, nodes);
             advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
-            store_rec(s, p, r);
+            store_rec32(s, p, r);
         }
     }
     __syncthreads();

@@ -103,6 +103,33 @@ __device__ inline void load_rec(const Species<real> &s, int64_t p, PRec<real> &r
     r.q = s.q[p];
 }
 
+// the same with a 32-bit slot: each array is its base (scalar registers)
+// plus one 32-bit byte offset shared by all of them
+template <typename T>
+__device__ inline T &at32(T *base, uint32_t byte_off)
+{
+    return *(T *)((char *)base + byte_off);
+}
+
+template <typename real>
+__device__ inline void load_rec32(const Species<real> &s, uint32_t p, PRec<real> &r)
+{
+    const uint32_t o4 = p * 4u, orl = p * (uint32_t)sizeof(real);
+    r.cell = at32(s.cell, o4);
+    r.o[0] = at32(s.ox, orl); r.o[1] = at32(s.oy, orl); r.o[2] = at32(s.oz, orl);
+    r.u[0] = at32(s.ux, orl); r.u[1] = at32(s.uy, orl); r.u[2] = at32(s.uz, orl);
+    r.q = at32(s.q, orl);
+}
+
+template <typename real>
+__device__ inline void store_rec32(const Species<real> &s, uint32_t p, const PRec<real> &r)
+{
+    const uint32_t o4 = p * 4u, orl = p * (uint32_t)sizeof(real);
+    at32(s.cell, o4) = r.cell;
+    at32(s.ox, orl) = r.o[0]; at32(s.oy, orl) = r.o[1]; at32(s.oz, orl) = r.o[2];
+    at32(s.ux, orl) = r.u[0]; at32(s.uy, orl) = r.u[1]; at32(s.uz, orl) = r.u[2];
+}
+
 // write back what a push changes (the charge does not change)
 template <typename real>
 __device__ inline void store_rec(const Species<real> &s, int64_t p, const PRec<real> &r)
