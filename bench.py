@@ -125,7 +125,19 @@ def field_error(w, steps):
     return out
 
 
-def cpu_baseline(w, budget_steps=2):
+def cpu_model():
+    """The host CPU's model name (the reference CPU path runs there)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(w, budget_steps=3):
     """The reference's own step (oracle/_ref/quill_ref: the reference's
     src/*.cpp units, single-threaded as the reference runs per rank) on the
     same workload: full grid, one warm-up step, then `budget_steps` timed
@@ -158,7 +170,7 @@ def cpu_baseline(w, budget_steps=2):
         return dict(value=rate, unit="particle-steps/s", cores=cores, kind="reference",
                     sample=f"reference src/*.cpp step (oracle/_ref), full {nx}x{ny}x{nz} grid, "
                            f"{rec['particles']} particles, 1 warm-up + {rec['steps']} timed steps, "
-                           f"{rec['seconds']:.2f} s, single thread")
+                           f"{rec['seconds']:.2f} s, single thread of {cpu_model()} ({os.cpu_count()} threads)")
     o = oracle.Oracle(nx, ny, nz, dx, dx, dx, dt, solver=solver, pusher=pusher)
     n = o.load_uniform(PPC, w["density"], SIGMA_U, SEED)
     o.laser_cos(*L)
@@ -168,7 +180,8 @@ def cpu_baseline(w, budget_steps=2):
     o.step(budget_steps)
     sec = time.perf_counter() - t0
     return dict(value=n * budget_steps / sec, unit="particle-steps/s", cores=cores, kind="port",
-                sample=f"FP64 C restatement (oracle/), full grid, {n} particles, {budget_steps} steps, {sec:.2f} s")
+                sample=f"FP64 C restatement (oracle/), full grid, {n} particles, {budget_steps} steps, {sec:.2f} s, "
+                       f"single thread of {cpu_model()}")
 
 
 def copy_calibration(nbytes):
@@ -212,6 +225,61 @@ def measured_traffic(workload):
                      vals["WRITE_SIZE_KiB_per_dispatch"] * vals["WRITE_SIZE_factor"])
 
 
+def adjoint_check(steps, every):
+    """The differentiable laser-plasma case at 128 x 64 x 64 cells, 8 per cell
+    (3.72M electrons), density ramp of 32 nodes about 0.005 n_cr, a0 = 2 pulse
+    with waist, chirp and 16 transverse-phase coefficients, NDFX + Vay,
+    `steps` steps with a snapshot every `every`: the FP32 adjoint gradient of
+    the electrons' kinetic energy above 5 mc^2 against central differences of
+    FP64 forward runs along random parameter directions, and against the FP64
+    adjoint gradient."""
+    import torch   # the adjoint's host side is torch; the plain bench needs none
+    from quill_amd.adjoint import LaserPlasma
+
+    def log(msg):
+        print(f"adjoint check: {msg}", file=sys.stderr, flush=True)
+
+    lp = LaserPlasma(steps=steps, every=every)
+    log(f"{len(lp.pos)} electrons, {steps} steps")
+    t = time.perf_counter()
+    L32, g32 = lp.gradient("fp32")
+    t32 = time.perf_counter() - t
+    log(f"FP32 forward + adjoint {t32:.1f} s")
+    t = time.perf_counter()
+    L64, g64 = lp.gradient("fp64")
+    t64 = time.perf_counter() - t
+    log(f"FP64 forward + adjoint {t64:.1f} s")
+    dots = []
+    for seed, keep in ((5, None), (6, ("nodes",)), (7, ("a0", "ysigma", "chirp", "c"))):
+        v = lp.direction(seed)
+        if keep:
+            v = {k: x for k, x in v.items() if k in keep}
+        fd = {}
+        for h in (1e-4, 1e-5):
+            fd[h] = lp.central_difference(v, h)
+            log(f"direction {keep or 'all'}, h = {h}: {fd[h]:.9e}")
+        a32 = sum(float((g32[k] * v[k]).sum()) for k in v)
+        a64 = sum(float((g64[k] * v[k]).sum()) for k in v)
+        dots.append({"direction": "+".join(keep) if keep else "all", "fp64_central_difference": fd,
+                     "fp32_adjoint": a32, "fp64_adjoint": a64,
+                     "fp32_rel_err": abs(a32 - fd[1e-5]) / abs(fd[1e-5]),
+                     "fp64_rel_err": abs(a64 - fd[1e-5]) / abs(fd[1e-5])})
+    a = torch.cat([g64[k].reshape(-1) for k in g64])
+    b = torch.cat([g32[k].reshape(-1) for k in g32])
+    nx, ny, nz = lp.shape
+    return {"metric": "adjoint gradient error vs FP64 central differences (max over directions)",
+            "value": max(d["fp32_rel_err"] for d in dots), "unit": "relative", "higher_is_better": False,
+            "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"laser-plasma-adjoint-{nx}x{ny}x{nz}", "particles": len(lp.pos), "ppc": lp.ppc,
+                       "steps": steps, "checkpoint_every": every, "solver": lp.solver, "pusher": lp.pusher,
+                       "density_nodes": len(lp.theta0["nodes"]), "phase_coefficients": len(lp.theta0["c"]),
+                       "objective": "sum |q| softplus(gamma - 6) over electrons (kinetic energy above 5 mc^2)"},
+            "objective_fp32": L32, "objective_fp64": L64,
+            "seconds_fp32_forward_adjoint": t32, "seconds_fp64_forward_adjoint": t64,
+            "fp32_vs_fp64_gradient_rel_l2": float((a - b).norm() / a.norm()),
+            "directions": dots}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -223,7 +291,13 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=50, help="steps of the plain FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison runs")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only run the adjoint check of the laser-plasma case (128x64x64, 300 steps)")
+    ap.add_argument("--adjoint-steps", type=int, default=300)
     args = ap.parse_args()
+    if args.adjoint:
+        print(json.dumps(adjoint_check(args.adjoint_steps, 25)), flush=True)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

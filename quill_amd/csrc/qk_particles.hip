@@ -215,29 +215,41 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
         }
     };
     const int64_t end = (int64_t)ch.y + ch.z;
-    // the records of the next two rounds are in flight while one is
-    // processed; slots as 32-bit indices: the record arrays are addressed as
-    // scalar base + one shared 32-bit byte offset (capacity keeps slot * size
-    // below 2^32)
-    PRec<real> nxt, nxt2;
+    // software pipeline over the rounds of kTBlock particles: while one
+    // particle is pushed, the next round's record has its field nodes
+    // gathered and the record two rounds ahead is loading.  Slots as 32-bit
+    // indices: the record arrays are addressed as scalar base + one shared
+    // 32-bit byte offset (capacity keeps slot * size below 2^32)
     uint32_t p = (uint32_t)ch.y + threadIdx.x;
     const uint32_t end32 = (uint32_t)end;
-    if (p < end32) load_rec32(s, p, nxt);
-    if (p + kTBlock < end32) load_rec32(s, p + kTBlock, nxt2);
+    PRec<real> r1, r2;   // the records one and two rounds ahead
+    if (p < end32) load_rec32(s, p, r1);
+    if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r2);
+    auto gather = [&](const PRec<real> &q, int &a, int &b, int &cc, bool &w, Corners<real> &nd) {
+        if (q.cell < 0) return;
+        a = cell_i(q.cell) - i0;
+        b = cell_j(q.cell) - j0;
+        cc = cell_k(q.cell) - k0;
+        w = (unsigned)(a + kHalo) < (unsigned)(kTx + 2 * kHalo) && (unsigned)(b + kHalo) < (unsigned)(kTy + 2 * kHalo) &&
+            (unsigned)(cc + kHalo) < (unsigned)(kTz + 2 * kHalo);
+        if (w) load_corners_stage(stage, a, b, cc, nd);
+        else load_corners(f, cell_base(g, q.cell), g.sx, g.sy, nd);
+    };
+    int ri1 = 0, rj1 = 0, rk1 = 0;
+    bool w1 = false;
+    Corners<real> nodes1;
+    if (p < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
     for (; p < end32; p += kTBlock) {
-        PRec<real> r = nxt;
-        nxt = nxt2;
-        if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, nxt2);
+        PRec<real> r = r1;
+        Corners<real> nodes = nodes1;
+        ri = ri1;
+        rj = rj1;
+        rk = rk1;
+        in_win = w1;
+        r1 = r2;
+        if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
+        if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, r2);
         if (r.cell >= 0) {
-            ri = cell_i(r.cell) - i0;
-            rj = cell_j(r.cell) - j0;
-            rk = cell_k(r.cell) - k0;
-            in_win = (unsigned)(ri + kHalo) < (unsigned)(kTx + 2 * kHalo) &&
-                     (unsigned)(rj + kHalo) < (unsigned)(kTy + 2 * kHalo) &&
-                     (unsigned)(rk + kHalo) < (unsigned)(kTz + 2 * kHalo);
-            Corners<real> nodes;
-            if (in_win) load_corners_stage(stage, ri, rj, rk, nodes);
-            else load_corners(f, cell_base(g, r.cell), g.sx, g.sy, nodes);
             advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
             store_rec32(s, p, r);
         }

@@ -374,8 +374,16 @@ __global__ void k_write_chunks(const int *__restrict__ tile_start, const int *__
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     const int s = tile_start[t], e = tile_start[t + 1];
-    int o = chunk_off[t];
-    for (int p = s; p < e && o < max_chunks; p += kChunk, ++o) chunks[o] = make_int4(t, p, min(kChunk, e - p), 0);
+    const int nc = (e - s + kChunk - 1) / kChunk;
+    if (nc == 0) return;
+    // equal parts: a tile a little over kChunk gives two halves, not a full
+    // chunk and a nearly empty one that pays the tile's staging again
+    const int sz = (e - s + nc - 1) / nc;
+    const int o = chunk_off[t];
+    for (int c = 0; c < nc && o + c < max_chunks; ++c) {
+        const int p = s + c * sz;
+        chunks[o + c] = make_int4(t, p, max(0, min(sz, e - p)), 0);
+    }
 }
 
 inline unsigned nblocks(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
