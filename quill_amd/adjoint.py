@@ -186,12 +186,14 @@ class LaserPlasma:
     per is_inside cell at positions uniform in each cell from `seed`, start at
     rest with charges from a piecewise-linear density of `nodes` equally
     spaced over the box (ramp_charges); the ions are an immobile background.
+    Electrons are loaded `margin` cells or more away from the layers where
+    particles are deleted.
     The objective is the final electrons' kinetic energy above `threshold`
     mc^2 (softplus_energy).  Lengths in lambda, densities in n_cr."""
 
     def __init__(self, shape=(128, 64, 64), ppc=8, steps=300, every=25, dx_lambda=0.05, dt_lambda=0.025,
                  solver="ndfx", pusher="vay", density=0.005, n_nodes=32, a0=2.0, waist=0.8, duration=1.0,
-                 x0=1.8, chirp=0.02, threshold=5.0, width=1.0, seed=7):
+                 x0=1.8, chirp=0.02, threshold=5.0, width=1.0, seed=7, margin=0):
         self.shape, self.ppc, self.steps, self.every = tuple(shape), ppc, steps, every
         self.d = dx_lambda * 2 * np.pi
         self.dt = dt_lambda * 2 * np.pi
@@ -199,8 +201,11 @@ class LaserPlasma:
         self.threshold, self.width = threshold, width
         nx, ny, nz = self.shape
         rng = np.random.default_rng(seed)
-        # every is_inside cell (1 <= i < nx - 2, likewise j, k), ppc positions each
-        i, j, k = np.meshgrid(np.arange(1, nx - 2), np.arange(1, ny - 2), np.arange(1, nz - 2), indexing="ij")
+        # every is_inside cell (1 <= i < nx - 2, likewise j, k) at least
+        # `margin` cells from the deleting layers, ppc positions each
+        m = 1 + margin
+        i, j, k = np.meshgrid(np.arange(m, nx - 1 - m), np.arange(m, ny - 1 - m), np.arange(m, nz - 1 - m),
+                              indexing="ij")
         cells = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).repeat(ppc, 0).astype(np.float64)
         self.pos = torch.from_numpy(cells + rng.uniform(0, 1, cells.shape))
         lam = 2 * np.pi
