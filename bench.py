@@ -76,22 +76,25 @@ def laser_params(w, dx):
 # cell-tile sort cadence of the FP32 run: every 10 steps measured fastest at
 # 256x128x128 (5: sorts dominate; 20: 1.3 % of pushes leave their tile window)
 SORT_EVERY = 10
+# ... and sooner when more than 1 % of a species was outside its tile window
+# (a strong laser: at a0 = 4 on 512x256x256 a fixed cadence of 10 leaves 3-10 %)
+SORT_FALLBACK = 0.01
 
 
-def make_region(w, precision, deposit, sort_every=SORT_EVERY):
+def make_region(w, precision, deposit, sort_every=SORT_EVERY, sort_fallback=SORT_FALLBACK):
     nx, ny, nz = w["grid"]
     dx = DX_LAMBDA * TWO_PI
     dt = DT_LAMBDA * TWO_PI
     r = qa.Region(nx, ny, nz, dx, dx, dx, dt, solver=w["solver"], pusher=w["pusher"], precision=precision,
-                  deposit=deposit, sort_every=sort_every, device=0)
+                  deposit=deposit, sort_every=sort_every, sort_fallback=sort_fallback, device=0)
     n = r.load_uniform_plasma(qa.ELECTRON, PPC, w["density"], SIGMA_U, SEED)
     r.init_laser_cos(**laser_params(w, dx))
     r.init_boundaries()
     return r, n
 
 
-def timed_run(w, precision, deposit, steps, warmup, sort_every=SORT_EVERY):
-    r, n0 = make_region(w, precision, deposit, sort_every)
+def timed_run(w, precision, deposit, steps, warmup, sort_every=SORT_EVERY, sort_fallback=SORT_FALLBACK):
+    r, n0 = make_region(w, precision, deposit, sort_every, sort_fallback)
     if warmup:
         r.step(warmup)
     r.sync()
@@ -364,7 +367,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "grid": [nx, ny, nz], "cells": ncell, "ppc": PPC,
                    "particles": main_run["n_start"], "solver": w["solver"], "pusher": w["pusher"], "qed": False,
-                   "deposit": "tiled", "sort_every": SORT_EVERY, "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
+                   "deposit": "tiled", "sort_every": SORT_EVERY,
+                   "sort_fallback": SORT_FALLBACK, "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_push (gather+push+deposit)", "avg_launch_ms": push_ms,

@@ -79,6 +79,8 @@ typedef struct {
     double norm_ep;               /* compute_energy particle multiplier (0 -> 1) */
     int32_t merging;              /* pmerging after a step that crosses the limit: 0 off, 1 "ti", 2 "nl" */
     double crnp;                  /* the limit: crpc x cells (src/main.cpp:1701), macro-particles per species */
+    double sort_fallback;         /* also sort a species when more than this fraction of it was outside its
+                                     tile's window two steps before (0: sort_every only) */
 } qk_params;
 
 /* per-step diagnostics, compute_N / compute_energy (src/fout.cpp:41-138) */

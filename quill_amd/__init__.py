@@ -51,7 +51,7 @@ class Params(C.Structure):
                 ("seed", C.c_uint64), ("capacity", C.c_int64),
                 ("n_ion", C.c_int32), ("ion_cmr", C.c_double * MAX_ION),
                 ("norm_n", C.c_double), ("norm_ef", C.c_double), ("norm_ep", C.c_double),
-                ("merging", C.c_int32), ("crnp", C.c_double)]
+                ("merging", C.c_int32), ("crnp", C.c_double), ("sort_fallback", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -136,7 +136,7 @@ class Region:
     def __init__(self, nx, ny, nz, dx, dy, dz, dt, solver="fdtd", pusher="boris", precision="fp32",
                  deposit="tiled", qed=False, sort_every=0, track_ids=False, device=0, e_s=1e10,
                  external_bz=0.0, q_vac=0.0, seed=1, capacity=0, ion_cmr=(), norm_n=1.0, norm_ef=1.0,
-                 norm_ep=1.0, merging=None, crnp=0.0):
+                 norm_ep=1.0, merging=None, crnp=0.0, sort_fallback=0.0):
         L = load_library()
         p = Params()
         p.nx, p.ny, p.nz = nx, ny, nz
@@ -158,6 +158,7 @@ class Region:
         p.norm_n, p.norm_ef, p.norm_ep = norm_n, norm_ef, norm_ep
         p.merging = {None: 0, "off": 0, "ti": 1, "nl": 2}[merging]
         p.crnp = crnp
+        p.sort_fallback = sort_fallback
         self.params = p
         self.shape = (nx, ny, nz)
         self.precision = precision

@@ -764,13 +764,28 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
             if (tp.steps > 0 && tp.steps % tp.every == 0) QK_TRY(tape_snapshot(c));
             ++tp.steps;
         }
-        // cell-tile sort: the cadence, or an unsorted tail (appended particles,
-        // pushed by the global kernel meanwhile) above 1/16 of the sorted part
-        // (photons only with QED, where pair creation leaves free slots)
+        // The window fallback of step l - 2: once step l - 1 has begun its
+        // particle phase, step l - 2 and its diagnostics copy are done, and
+        // the GPU still has step l - 1 queued while the host decides.
+        bool fb_high[QK_MAX_SPECIES] = {};
+        if (tiled && c->p.sort_fallback > 0 && l >= 2) {
+            HIP_TRY(hipEventSynchronize(c->ev[3 * (l - 1)]));
+            const double *h = hd + (size_t)(l - 2) * kDiagLen;
+            for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+                const double live = h[s * kDiagPerSpecies + 2], fb = h[s * kDiagPerSpecies + 3];
+                fb_high[s] = live > 0 && fb > c->p.sort_fallback * live;
+            }
+        }
+        // cell-tile sort: the cadence, a fallback above sort_fallback (seen
+        // on a layout at least two steps old), or an unsorted tail (appended
+        // particles, pushed by the global kernel meanwhile) above 1/16 of the
+        // sorted part (photons only with QED, where pair creation leaves free
+        // slots)
         for (int s = 0; s < QK_MAX_SPECIES; ++s) {
             SpeciesBuf &b = c->sp[s];
             if (!b.active || b.n == 0 || (b.kind == 1 && !qed)) continue;
-            const bool due = c->p.sort_every > 0 && b.steps_since_sort >= c->p.sort_every;
+            const bool due = (c->p.sort_every > 0 && b.steps_since_sort >= c->p.sort_every) ||
+                             (fb_high[s] && b.steps_since_sort >= 2);
             const bool tail = tiled && (b.n_sorted == 0 || (b.n - b.n_sorted) * 16 > b.n_sorted);
             if (tail || due) QK_TRY(sort_species<real>(c, s));
         }
@@ -925,6 +940,7 @@ int qk_create(const qk_params *params, qk_ctx **out)
     if (p.deposit != QK_DEPOSIT_TILED && p.deposit != QK_DEPOSIT_GLOBAL) return fail(QK_ERR_INVALID, "unknown deposit");
     if (p.n_ion < 0 || p.n_ion > QK_MAX_ION) return fail(QK_ERR_INVALID, "n_ion out of range");
     if (p.sort_every < 0) return fail(QK_ERR_INVALID, "sort_every must be >= 0");
+    if (!(p.sort_fallback >= 0 && p.sort_fallback <= 1)) return fail(QK_ERR_INVALID, "sort_fallback must be in [0, 1]");
     if (p.qed && !(p.e_s > 0)) return fail(QK_ERR_INVALID, "QED needs e_s > 0");
     if (p.merging < 0 || p.merging > 2) return fail(QK_ERR_INVALID, "merging must be 0, 1 (ti) or 2 (nl)");
     if (p.merging && !(p.crnp > 0)) return fail(QK_ERR_INVALID, "merging needs crnp > 0");

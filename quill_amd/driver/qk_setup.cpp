@@ -464,6 +464,7 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
     p.precision = QK_FP32;
     p.deposit = QK_DEPOSIT_TILED;
     p.sort_every = 20;
+    p.sort_fallback = 0.01;   // and earlier when a laser sweeps particles out of their tiles' windows
     // merging and its limit crnp = crpc x the box in cells (src/main.cpp:1699-1701, 3397-3401)
     {
         const std::string &pm = c.get("pmerging").units;
