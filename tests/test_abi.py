@@ -34,9 +34,9 @@ def test_struct_layout_matches_c():
 #include <stddef.h>
 #include "quillk.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(qk_params), offsetof(qk_params, dx),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(qk_params), offsetof(qk_params, dx),
          offsetof(qk_params, e_s), offsetof(qk_params, seed), offsetof(qk_params, ion_cmr),
-         offsetof(qk_params, norm_ep), sizeof(qk_diag));
+         offsetof(qk_params, norm_ep), offsetof(qk_params, sort_fallback), sizeof(qk_diag));
   return 0;
 }
 """
@@ -48,7 +48,7 @@ int main(void) {
         got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     P = qa.Params
     want = [C.sizeof(P), P.dx.offset, P.e_s.offset, P.seed.offset, P.ion_cmr.offset, P.norm_ep.offset,
-            C.sizeof(qa.Diag)]
+            P.sort_fallback.offset, C.sizeof(qa.Diag)]
     assert got == want
 
 
@@ -60,6 +60,8 @@ def test_create_rejects_bad_config_without_crashing():
         qa.Region(8, 8, 8, 0.3, 0.3, 0.3, 0.5)
     with pytest.raises(qa.QuillError):
         qa.Region(8, 8, 8, 0.3, 0.3, 0.3, 0.1, solver=7)
+    with pytest.raises(qa.QuillError, match="sort_fallback"):
+        qa.Region(8, 8, 8, 0.3, 0.3, 0.3, 0.1, sort_fallback=1.5)
 
 
 def test_null_arguments_return_errors():

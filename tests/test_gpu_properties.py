@@ -84,10 +84,10 @@ def run_pair(nx, ny, nz, solver, pusher, nsteps, sigma, ppc=2, seed=3):
     return o, (E0, B0, rec, ids)
 
 
-def gpu_region(inputs, shape, solver, pusher, precision, deposit, sort_every=4):
+def gpu_region(inputs, shape, solver, pusher, precision, deposit, sort_every=4, sort_fallback=0.0):
     E0, B0, rec, ids = inputs
     r = qa.Region(*shape, DX, DX, DX, DT, solver=solver, pusher=pusher, precision=precision, deposit=deposit,
-                  sort_every=sort_every, track_ids=True)
+                  sort_every=sort_every, sort_fallback=sort_fallback, track_ids=True)
     r.set_fields(E0, B0)
     r.add_records(rec, ids)
     r.init_boundaries()
@@ -117,6 +117,30 @@ def test_fp64_tiled_matches_oracle(solver, pusher):
     np.testing.assert_array_equal(ids[go], oids[oo])
     np.testing.assert_allclose(rec[go][:, :6], orec[oo][:, :6], rtol=1e-9, atol=1e-10)
     assert 0 <= frac < 0.5
+
+
+def test_adaptive_sort_matches_oracle():
+    """sort_fallback: with no fixed cadence, fast particles (u ~ 0.6) leave
+    their tile windows within a few steps; the host sorts a species as soon
+    as more than 1 % of it was outside, which keeps the fallback low, and the
+    FP64 result still matches the oracle."""
+    shape = (40, 32, 24)
+    nsteps = 16
+    o, inputs = run_pair(*shape, "fdtd", "vay", nsteps, sigma=0.6)
+    o.step(nsteps)
+    frac = {}
+    for fb in (0.0, 0.01):
+        print(f"fp64 tiled, sort_every 0, sort_fallback {fb}", flush=True)
+        r = gpu_region(inputs, shape, "fdtd", "vay", "fp64", "tiled", sort_every=0, sort_fallback=fb)
+        r.step(nsteps)
+        E, B, J, _ = r.get_fields()
+        frac[fb] = r.fallback_fraction()
+        r.close()
+        for name, a, b in (("E", E, o.field(oracle.E)), ("B", B, o.field(oracle.B)), ("J", J, o.field(oracle.J))):
+            err = np.abs(a - b).max() / np.abs(b).max()
+            assert err < 1e-10, f"sort_fallback {fb}, {name}: {err:.3e}"
+    assert frac[0.0] > 0.05
+    assert frac[0.01] < 0.5 * frac[0.0]
 
 
 def test_fp32_tiled_equals_global_deposit():
