@@ -439,7 +439,11 @@ def main():
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
                      "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
                      "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "cell_updates_per_s": ncell * K / main_run["seconds"],
+        # field-only rate: the four field kernels of a step (B half step,
+        # E step, B half step, interpolate_be) over their own time; at FP32
+        # they move 36 + 60 + 36 + 24 = 156 B per cell
+        "field_cell_updates_per_s": ncell * K / (main_run["ms_fields"] * 1e-3),
+        "field_phase_GBs": 156.0 * ncell * K / (main_run["ms_fields"] * 1e-3) / 1e9,
         "ms_particles_per_step": push_ms,
         "ms_fields_per_step": main_run["ms_fields"] / K,
         "ms_sort_and_launch_per_step": ms_step - main_run["ms_events"] / K,
