@@ -11,7 +11,7 @@ inputs (the FP64 reference gradient):
     must match the oracle's difference quotient to round-off, for all three
     solvers (the transposes of the B and E updates and of interpolate_be);
   * electrons, positrons and ions in random fields: FP64 to 1e-6 relative,
-    FP32 to 1e-3, with snapshots every 3 steps so the
+    FP32 to 1e-3 (the stated bar for FP32 gradients), with snapshots every 3 steps so the
     segment recomputation is exercised.
 The step is piecewise smooth (face crossings switch the deposit's pieces);
 the perturbation is small enough not to cross a kink in these cases, and no
@@ -151,7 +151,7 @@ def test_fp64_adjoint_matches_oracle_differences(solver, pusher):
 
 
 @pytest.mark.parametrize("solver, pusher", [("fdtd", "boris"), ("ndfx", "vay")])
-def test_fp32_adjoint_within_north_star_bar(solver, pusher):
+def test_fp32_adjoint_within_1e_3(solver, pusher):
     check(31, solver, pusher, "fp32", 8, 3, True, 1e-3)
 
 
