@@ -6,7 +6,7 @@
 // src/main.cpp:160-187).
 //
 //   quillk-run CONF [--steps N] [--fp64] [--deposit tiled|global]
-//              [--sort-every N] [--device D] [--out DIR] [--dry-run]
+//              [--sort-every N] [--sort-fallback F] [--device D] [--out DIR] [--dry-run]
 //
 // Output iterations write the reference's plane files (ex .. bz, w, inv,
 // rho, rho_p, rho_ph, jx .. jz as the config asks) unless --no-output.
@@ -42,7 +42,8 @@ int usage()
 {
     std::fprintf(stderr,
                  "usage: quillk-run CONF [--steps N] [--fp64] [--deposit tiled|global] [--sort-every N]\n"
-                 "                  [--device D] [--out DIR] [--no-output] [--dry-run] [--print-conf]\n"
+                 "                  [--sort-fallback F] [--device D] [--out DIR] [--no-output] [--dry-run]\n"
+                 "                  [--print-conf]\n"
                  "                  [--dump-setup DIR] [--mw-fills NMW]\n"
                  "       quillk-run --write-planes IN DIR\n");
     return 2;
@@ -204,6 +205,7 @@ int main(int argc, char **argv)
     if (argc == 4 && std::strcmp(argv[1], "--write-planes") == 0) return write_planes(argv[2], argv[3]);
     std::string conf = argv[1], out_dir, setup_dir;
     int steps_override = -1, device = 0, sort_every = -1, fills_nmw = 0;
+    double sort_fallback = -1;
     bool fp64 = false, dry = false, global_dep = false, show_conf = false, write_outputs = true;
     for (int a = 2; a < argc; ++a) {
         const std::string s = argv[a];
@@ -216,6 +218,7 @@ int main(int argc, char **argv)
         else if (s == "--dump-setup") { const char *v = next(); if (!v) return usage(); setup_dir = v; }
         else if (s == "--deposit") { const char *v = next(); if (!v) return usage(); global_dep = std::strcmp(v, "global") == 0; }
         else if (s == "--sort-every") { const char *v = next(); if (!v) return usage(); sort_every = std::atoi(v); }
+        else if (s == "--sort-fallback") { const char *v = next(); if (!v) return usage(); sort_fallback = std::atof(v); }
         else if (s == "--device") { const char *v = next(); if (!v) return usage(); device = std::atoi(v); }
         else if (s == "--out") { const char *v = next(); if (!v) return usage(); out_dir = v; }
         else if (s == "--mw-fills") { const char *v = next(); if (!v) return usage(); fills_nmw = std::atoi(v); }
@@ -241,6 +244,7 @@ int main(int argc, char **argv)
     if (fp64) p.precision = QK_FP64;
     if (global_dep) p.deposit = QK_DEPOSIT_GLOBAL;
     if (sort_every >= 0) p.sort_every = sort_every;
+    if (sort_fallback >= 0) p.sort_fallback = sort_fallback;
     p.device = device;
     const int steps = steps_override >= 0 ? steps_override : run.last_iteration + 1;
     const std::vector<Species> sp = make_particles(run);
