@@ -20,6 +20,12 @@
  *                            the rate functions from the compiled reference
  *   QED event sequences      statistical only (the reference's generator is
  *                            seeded from the wall clock, mc.cpp:12-23)
+ *   moving window, max_w,    golden runs of the compiled reference with the
+ *   fill_cells               window on (fixed and automatic shifts)
+ *   pmerging                 statistics of 16 merges by the compiled reference
+ *   compute_rho              the Gauss-law invariant div E - rho + golden planes
+ *   uniform plasma loader    no reference counterpart (the builder's Philox
+ *                            loader); the GPU loader must match it exactly
  */
 #include "quill_oracle.h"
 #include "philox.h"
