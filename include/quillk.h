@@ -173,15 +173,18 @@ int qk_fill_cells(qk_ctx *ctx, int species, int64_t ncells, const int32_t *cells
  * trajectory that the following qk_step calls make: a snapshot of the whole
  * state now and every `every` steps (memory: one state per snapshot plus
  * `every` states while the adjoint runs).  It needs qed = 0, merging = 0,
- * deposit = QK_DEPOSIT_GLOBAL and sort_every = 0 (particles keep their slots)
- * and no photons.  No counterpart in the reference: it differentiates the
- * loop body of src/main.cpp:2319-2332 (padvance and fadvance). */
+ * sort_every = 0, sort_fallback = 0 (particles keep their slots) and no
+ * photons.  With QK_DEPOSIT_TILED it sorts the particles into tiles once,
+ * first, which permutes the slots (read the new order with qk_get_particles
+ * and ids).  No counterpart in the reference: it differentiates the loop
+ * body of src/main.cpp:2319-2332 (padvance and fadvance). */
 int qk_tape_begin(qk_ctx *ctx, int every);
 /* Given the adjoints of the state after the recorded steps -- lam_E, lam_B
  * (AoS [nx][ny][nz][3], may be NULL for zero) and per species slot
  * (QK_MAX_SPECIES pointers, each NULL or n x 7 doubles: x y z ux uy uz q of
- * the n slots the species had at qk_tape_begin, in the order they were
- * added; zero for particles deleted on the way) -- return the adjoints of
+ * the n slots the species had after qk_tape_begin, in slot order -- the
+ * order they were added with QK_DEPOSIT_GLOBAL; zero for particles deleted
+ * on the way) -- return the adjoints of
  * the state at qk_tape_begin in grad_E, grad_B (before init_boundaries: zero
  * on the planes it clears) and grad_part (same layout).  Positions are in
  * cells.  The state after the call is the state before it. */

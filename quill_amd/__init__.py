@@ -297,16 +297,25 @@ class Region:
 
     # -- adjoint ------------------------------------------------------------
     def tape_begin(self, every):
-        """Record the following steps for adjoint(): a snapshot now and every `every` steps."""
+        """Record the following steps for adjoint(): a snapshot now and every
+        `every` steps.  With the tiled deposit the particles are first sorted
+        into tiles, which permutes their slots; tape_ids() gives the order."""
         _check(load_library().qk_tape_begin(self._h, every), "qk_tape_begin")
         self._tape_n = {s: self.count(s) for s in range(ION0 + self.params.n_ion)}
+        self._tape_ids = {s: self.particles(s)[1] for s, n in self._tape_n.items() if n > 0} \
+            if self.params.track_ids else {}
+
+    def tape_ids(self, species):
+        """Ids of the particles in slot order after tape_begin (needs track_ids)."""
+        return self._tape_ids[species]
 
     def adjoint(self, lam_E=None, lam_B=None, lam_parts=None):
         """Adjoint of the recorded steps.  lam_E, lam_B: (nx, ny, nz, 3) adjoints
         of the final fields; lam_parts: {species: (n, 7)} adjoints of the final
-        x y z ux uy uz q of the n slots the species had at tape_begin (in the
-        order added; zero for deleted particles).  Returns (grad_E0, grad_B0,
-        {species: (n, 7)}) at tape_begin."""
+        x y z ux uy uz q of the n slots the species had after tape_begin, in
+        slot order (tape_ids; the order added with the global deposit; zero
+        for deleted particles).  Returns (grad_E0, grad_B0, {species: (n, 7)})
+        at tape_begin, in the same order."""
         L = load_library()
         shape = self.shape + (3,)
         lE = np.ascontiguousarray(lam_E, np.float64) if lam_E is not None else None

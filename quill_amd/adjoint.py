@@ -8,7 +8,9 @@ per-particle adjoint of gather, push and charge-conserving deposit, swept
 backwards over a trajectory kept as snapshots every `every` steps and
 recomputed one segment at a time.  The differentiated loop body is
 src/main.cpp:2319-2332 (padvance, src/advance.cpp:90-387, and fadvance,
-src/advance.cpp:13-83) with QED, merging and sorting off.
+src/advance.cpp:13-83) with QED and merging off; with the tiled deposit
+(the default here) the particles are sorted into tiles once, at
+qk_tape_begin, and keep their slots after that.
 
 Tensors cross the boundary on the host in FP64, as the C ABI takes them;
 the arithmetic runs on the GPU in the context's precision.
@@ -26,7 +28,8 @@ class RunSpec:
     """What one differentiable run does.
 
     region: keyword arguments of Region (grid, steps, solver, pusher,
-    precision, ion_cmr, ...); deposit, sort_every and track_ids are set here.
+    precision, ion_cmr, deposit, ...); sort_every, sort_fallback and
+    track_ids are set here, deposit defaults to "tiled".
     species: the species slots whose particles are passed, in order.
     """
     region: dict
@@ -41,7 +44,8 @@ def simulate(spec, E0, B0, parts, tape=False):
     them).  Returns (region, E, B, [particles]) with FP64 tensors; the region
     is open, recorded for qk_adjoint when `tape`, and closed otherwise."""
     kw = dict(spec.region)
-    kw.update(deposit="global", sort_every=0, track_ids=True)
+    kw.update(sort_every=0, sort_fallback=0.0, track_ids=True)
+    kw.setdefault("deposit", "tiled")
     r = Region(**kw)
     try:
         r.set_fields(E0.detach().cpu().double().numpy(), B0.detach().cpu().double().numpy())
@@ -89,22 +93,32 @@ class PICRun(torch.autograd.Function):
         r, E, B, outs = simulate(spec, E0, B0, parts, tape=True)
         ctx.region = r
         ctx.spec = spec
+        ctx.shapes = [tuple(P.shape) for P in parts]
         return (E, B, *outs)
 
     @staticmethod
     def backward(ctx, gE, gB, *gparts):
         spec, r = ctx.spec, ctx.region
-        lam = {}
+        # rows are ids (the order added); the library's adjoint works in the
+        # slot order after tape_begin
+        lam, order = {}, {s: r.tape_ids(s) for s in spec.species if s in r._tape_ids}
         for s, g in zip(spec.species, gparts):
-            if g is not None:
-                lam[s] = g.detach().cpu().double().numpy() * spec.alive[s][:, None]
+            if g is not None and s in order:
+                lam[s] = (g.detach().cpu().double().numpy() * spec.alive[s][:, None])[order[s]]
         try:
             gE0, gB0, gp = r.adjoint(gE.detach().cpu().double().numpy() if gE is not None else None,
                                      gB.detach().cpu().double().numpy() if gB is not None else None, lam)
         finally:
             r.close()
-        return (None, torch.from_numpy(gE0), torch.from_numpy(gB0),
-                *[torch.from_numpy(gp[s]) if s in gp else None for s in spec.species])
+        grads = []
+        for s, P in zip(spec.species, ctx.shapes):
+            if s in gp:
+                g = np.zeros(P)
+                g[order[s]] = gp[s]
+                grads.append(torch.from_numpy(g))
+            else:
+                grads.append(None)
+        return (None, torch.from_numpy(gE0), torch.from_numpy(gB0), *grads)
 
 
 def run(spec, E0, B0, *parts):

@@ -1392,9 +1392,18 @@ int qk_tape_begin(qk_ctx *c, int every)
     if (!c) return fail(QK_ERR_INVALID, "null context");
     if (every <= 0) return fail(QK_ERR_INVALID, "checkpoint interval must be positive");
     const qk_params &p = c->p;
-    if (p.qed || p.merging || p.deposit != QK_DEPOSIT_GLOBAL || p.sort_every != 0)
-        return fail(QK_ERR_INVALID, "the adjoint needs qed = 0, merging = 0, deposit = QK_DEPOSIT_GLOBAL, sort_every = 0");
+    if (p.qed || p.merging || p.sort_every != 0 || p.sort_fallback != 0)
+        return fail(QK_ERR_INVALID, "the adjoint needs qed = 0, merging = 0, sort_every = 0, sort_fallback = 0");
     if (c->sp[QK_PHOTON].active && c->sp[QK_PHOTON].n > 0) return fail(QK_ERR_INVALID, "the adjoint covers charged species only");
+    // With the tiled deposit the particles are sorted into tiles once, here;
+    // no later sort runs (sort_every = 0, nothing is appended), so every
+    // particle keeps its slot from snapshot to snapshot.  The tiles' windows
+    // then age, and particles that leave them go through the kernel's global
+    // fallback.
+    if (p.deposit == QK_DEPOSIT_TILED)
+        for (int s = 0; s < QK_MAX_SPECIES; ++s)
+            if (c->sp[s].active && c->sp[s].n > 0 && c->sp[s].kind == 0)
+                QK_TRY(c->rsize == 4 ? sort_species<float>(c, s) : sort_species<double>(c, s));
     tape_free(c);
     Tape &t = c->tape;
     t.every = every;
