@@ -73,11 +73,11 @@ def laser_params(w, dx):
     return dict(a0y=w["a0"], a0z=0.0, xsigma=xsigma, ysigma=ysig, zsigma=ysig, x0=x0, b_sign=1, x1=x1)
 
 
-# cell-tile sort cadence of the FP32 run: every 10 steps measured fastest at
-# 256x128x128 (5: sorts dominate; 20: 1.3 % of pushes leave their tile window)
-SORT_EVERY = 10
-# ... and sooner when more than 1 % of a species was outside its tile window
-# (a strong laser: at a0 = 4 on 512x256x256 a fixed cadence of 10 leaves 3-10 %)
+# cell-tile sorts of the FP32 run: whenever more than 1 % of the electrons
+# were outside their tile windows, and at least every 40 steps.  Measured at
+# 256x128x128 over 500 steps: 1.011 ms per step, against 1.028 with a fixed
+# sort every 10 steps; thresholds of 0.5 % and 2 % gave 1.020 and 1.024 ms
+SORT_EVERY = 40
 SORT_FALLBACK = 0.01
 
 
