@@ -120,21 +120,38 @@ __global__ void __launch_bounds__(256) k_count_keys(const int *__restrict__ cell
 
 constexpr int kSortBlock = 256;
 
-// sorted position inside the tile -> its storage slot; copy the record
+// one record in registers, loaded before its destination is known
 template <typename real>
-__device__ inline void place_record(const Species<real> &src, const Species<real> &dst, int64_t p, int c, int o)
-{
-    dst.cell[o] = c;
-    dst.ox[o] = src.ox[p];
-    dst.oy[o] = src.oy[p];
-    dst.oz[o] = src.oz[p];
-    dst.ux[o] = src.ux[p];
-    dst.uy[o] = src.uy[p];
-    dst.uz[o] = src.uz[p];
-    dst.q[o] = src.q[p];
-    if (src.chi) dst.chi[o] = src.chi[p];
-    if (src.id) dst.id[o] = src.id[p];
-}
+struct HeldRecord {
+    real o[3], u[3], q, chi;
+    int id;
+    __device__ inline void load(const Species<real> &src, int64_t p)
+    {
+        o[0] = src.ox[p];
+        o[1] = src.oy[p];
+        o[2] = src.oz[p];
+        u[0] = src.ux[p];
+        u[1] = src.uy[p];
+        u[2] = src.uz[p];
+        q = src.q[p];
+        chi = src.chi ? src.chi[p] : real(0);
+        id = src.id ? src.id[p] : 0;
+    }
+    __device__ inline void store(const Species<real> &dst, int c, int at) const
+    {
+        dst.cell[at] = c;
+        dst.ox[at] = o[0];
+        dst.oy[at] = o[1];
+        dst.oz[at] = o[2];
+        dst.ux[at] = u[0];
+        dst.uy[at] = u[1];
+        dst.uz[at] = u[2];
+        dst.q[at] = q;
+        if (dst.chi) dst.chi[at] = chi;
+        if (dst.id) dst.id[at] = id;
+    }
+};
+
 
 __device__ inline int tile_of_cell(const TileGeom &tg, int c)
 {
@@ -252,6 +269,9 @@ __global__ void __launch_bounds__(kSortBlock) k_scatter_keep(Species<real> src, 
         const int p = b0 + threadIdx.x;
         const int c = p < e ? src.cell[p] : -1;
         const int tn = c >= 0 ? tile_of_cell(tg, c) : -1;
+        // the record's loads are in flight across the barriers below
+        HeldRecord<real> rec;
+        if (c >= 0) rec.load(src, p);
         const bool keep = tn == t;
         const unsigned long long m = __ballot(keep);
         const int before = __popcll(m & ((1ull << lane) - 1ull));
@@ -269,9 +289,9 @@ __global__ void __launch_bounds__(kSortBlock) k_scatter_keep(Species<real> src, 
         if (threadIdx.x < 27 && bins[threadIdx.x] > 0)
             bin_base[threadIdx.x] = atomicAdd(cur + bin_tile(tg, t, threadIdx.x), bins[threadIdx.x]);
         __syncthreads();
-        if (keep) place_record(src, dst, p, c, run + wbase + before);
-        else if (b >= 0) place_record(src, dst, p, c, bin_base[b] + rank);
-        else if (c >= 0) place_record(src, dst, p, c, atomicAdd(cur + tn, 1));
+        if (keep) rec.store(dst, c, run + wbase + before);
+        else if (b >= 0) rec.store(dst, c, bin_base[b] + rank);
+        else if (c >= 0) rec.store(dst, c, atomicAdd(cur + tn, 1));
         run += total;
         __syncthreads();   // wsum and the bins are rewritten next round
     }
@@ -286,6 +306,8 @@ __global__ void __launch_bounds__(256) k_scatter_tail(Species<real> src, Species
     const int64_t p = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int c = p < n ? src.cell[p] : -1;
     const int key = c >= 0 ? tile_of_cell(tg, c) : -1;
+    HeldRecord<real> rec;   // in flight while the position is claimed
+    if (c >= 0) rec.load(src, p);
     const int lane = threadIdx.x & 63;
     int head, len;
     key_runs(key, lane, head, len);
@@ -293,7 +315,7 @@ __global__ void __launch_bounds__(256) k_scatter_tail(Species<real> src, Species
     if (lane == head && key >= 0) base = atomicAdd(cur + key, len);
     base = __shfl(base, head, 64);
     if (c < 0) return;
-    place_record(src, dst, p, c, base + (lane - head));
+    rec.store(dst, c, base + (lane - head));
 }
 
 // The scatter reads its source in XCD-sized contiguous ranges (xcd_block):
@@ -308,6 +330,8 @@ __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real
     const int64_t p = first + xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int c = p < n ? src.cell[p] : -1;
     const int key = c >= 0 ? sort_key(tg, c) : -1;
+    HeldRecord<real> rec;   // in flight while the position is claimed
+    if (c >= 0) rec.load(src, p);
     const int lane = threadIdx.x & 63;
     int head, len;
     key_runs(key, lane, head, len);
@@ -318,7 +342,7 @@ __global__ void __launch_bounds__(256) k_scatter(Species<real> src, Species<real
     // sorted position inside the tile -> its spread storage slot
     const int t = key / kTileCells;
     const int ts = tile_start[t], nt = tile_start[t + 1] - ts;
-    place_record(src, dst, p, c, ts + (int)((int64_t)(base + (lane - head) - ts) * tile_mult[t] % nt));
+    rec.store(dst, c, ts + (int)((int64_t)(base + (lane - head) - ts) * tile_mult[t] % nt));
 }
 
 // Storage order inside a tile.  The n particles of a tile, sorted by cell,
