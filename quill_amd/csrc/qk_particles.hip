@@ -186,6 +186,14 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
     const int tk = tile % tg.ntz, t2 = tile / tg.ntz;
     const int tj = t2 % tg.nty, ti = t2 / tg.nty;
     const int i0 = ti * kTx - tg.ishift, j0 = tj * kTy, k0 = tk * kTz;
+    // Slots as 32-bit indices: the record arrays are addressed as scalar
+    // base + one shared 32-bit byte offset (capacity keeps slot * size below
+    // 2^32).  The first two rounds' records load while the tile is staged.
+    const uint32_t end32 = (uint32_t)(ch.y + ch.z);
+    uint32_t p = (uint32_t)ch.y + threadIdx.x;
+    PRec<real> r1, r2;   // the records one and two rounds ahead
+    if (p < end32) load_rec32(s, p, r1);
+    if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r2);
     for (int w = threadIdx.x; w < kJWin; w += kTBlock) win[w] = 0.0;
     {
         const int si = i0 - kHalo, sj = j0 - kHalo, sk = k0 - kHalo;
@@ -214,17 +222,10 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
             acc_d.fallback += 1;
         }
     };
-    const int64_t end = (int64_t)ch.y + ch.z;
     // software pipeline over the rounds of kTBlock particles: while one
     // particle is pushed, the next round's record has its field nodes
-    // gathered and the record two rounds ahead is loading.  Slots as 32-bit
-    // indices: the record arrays are addressed as scalar base + one shared
-    // 32-bit byte offset (capacity keeps slot * size below 2^32)
-    uint32_t p = (uint32_t)ch.y + threadIdx.x;
-    const uint32_t end32 = (uint32_t)end;
-    PRec<real> r1, r2;   // the records one and two rounds ahead
-    if (p < end32) load_rec32(s, p, r1);
-    if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r2);
+    // gathered and the record two rounds ahead is loading (the first two
+    // rounds' records were issued before the staging)
     auto gather = [&](const PRec<real> &q, int &a, int &b, int &cc, bool &w, Corners<real> &nd) {
         if (q.cell < 0) return;
         a = cell_i(q.cell) - i0;
