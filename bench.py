@@ -1,11 +1,13 @@
 """Benchmark of the per-step PIC loop on one MI355X.
 
-Workload (default "uniform-ndfx-vay-256x128x128"): 256 x 128 x 128 cells,
-dx = dy = dz = 0.05 lambda, dt = 0.025 lambda/c, NDFX solver, Vay pusher, QED
-off, uniform electron plasma at 0.01 n_cr with 8 particles per cell (is_inside
-cells only), thermal momenta N(0, 0.01^2), linearly polarised laser a0 = 1,
-waist 2 lambda, duration 4 lambda/c set as the reference's cos-envelope
-initial condition at x-min.  FP32 on the GPU.
+Headline workload "wakefield-fdtd-boris-512x256x256", the largest case that
+fits one GPU: 512 x 256 x 256 cells, dx = dy = dz = 0.05 lambda, dt = 0.025
+lambda/c, FDTD solver, Boris pusher, QED off, uniform electron plasma at
+0.001 n_cr with 8 particles per cell (is_inside cells only: 260.6 M
+electrons), thermal momenta N(0, 0.01^2), linearly polarised laser a0 = 4,
+waist 6 lambda, duration 8 lambda/c set as the reference's cos-envelope
+initial condition at x-min, cell-tile sort at least every 20 steps.  FP32 on
+the GPU.
 
 A "step" is one pass of src/main.cpp:2319-2332 (padvance, compute_N,
 compute_energy, fadvance) over the whole subdomain.  The timed region holds K
@@ -16,7 +18,10 @@ there).  torch is not imported: the library brings its own HIP runtime.
 Printed: one JSON line with value = particle-steps/s, the roofline of the
 dominant kernel (the fused gather/push/deposit kernel) from HIP events on the
 library's stream, the plain FP64 GPU method on the same workload, the FP32 vs
-FP64 field error, and the reference CPU path timed on this host.
+FP64 field error, and the reference CPU path timed on this host (on one
+GPU's x-slab of the grid, a bounded sample).  Two more blocks: the
+256 x 128 x 128 NDFX / Vay case with its 500-step FP32-vs-FP64 errors, and the
+64 x 32 x 32 FDTD / Boris case with the reference's own 200 steps on the CPU.
 """
 import argparse
 import json
@@ -74,14 +79,19 @@ def laser_params(w, dx):
 
 
 # cell-tile sorts of the FP32 run: whenever more than 1 % of the electrons
-# were outside their tile windows, and at least every 40 steps.  Measured at
+# were outside their tile windows, and at least every SORT_EVERY steps
+# (the large wakefield case asks for a sort every 20 steps).  Measured at
 # 256x128x128 over 500 steps: 1.011 ms per step, against 1.028 with a fixed
 # sort every 10 steps; thresholds of 0.5 % and 2 % gave 1.020 and 1.024 ms
-SORT_EVERY = 40
+SORT_EVERY = {"uniform-ndfx-vay-256x128x128": 40, "uniform-fdtd-boris-64x32x32": 20,
+              "wakefield-fdtd-boris-512x256x256": 20}
 SORT_FALLBACK = 0.01
+HEADLINE = "wakefield-fdtd-boris-512x256x256"
+# the CPU sample of the headline: one GPU's x-slab of an 8-way split
+CPU_SAMPLE_GRID = {"wakefield-fdtd-boris-512x256x256": (64, 256, 256)}
 
 
-def make_region(w, precision, deposit, sort_every=SORT_EVERY, sort_fallback=SORT_FALLBACK):
+def make_region(w, precision, deposit, sort_every=20, sort_fallback=SORT_FALLBACK):
     nx, ny, nz = w["grid"]
     dx = DX_LAMBDA * TWO_PI
     dt = DT_LAMBDA * TWO_PI
@@ -93,7 +103,7 @@ def make_region(w, precision, deposit, sort_every=SORT_EVERY, sort_fallback=SORT
     return r, n
 
 
-def timed_run(w, precision, deposit, steps, warmup, sort_every=SORT_EVERY, sort_fallback=SORT_FALLBACK):
+def timed_run(w, precision, deposit, steps, warmup, sort_every=20, sort_fallback=SORT_FALLBACK):
     r, n0 = make_region(w, precision, deposit, sort_every, sort_fallback)
     if warmup:
         r.step(warmup)
@@ -112,14 +122,14 @@ def timed_run(w, precision, deposit, steps, warmup, sort_every=SORT_EVERY, sort_
                 ms_particles=ms_part, ms_fields=ms_field, energy_f_last=diag[-1]["energy_f"], fallback=fallback)
 
 
-def field_error(w, steps, moments=False):
+def field_error(w, steps, moments=False, sort_every=20):
     """Per-component relative L2 error of FP32 vs the plain FP64 method after
     `steps`; with `moments`, also the relative errors of the field energy,
     the electrons' kinetic energy and charge (compute_energy / compute_N)."""
     out = {}
     res, dg = {}, {}
     for prec, dep in (("fp32", "tiled"), ("fp64", "global")):
-        r, _ = make_region(w, prec, dep)
+        r, _ = make_region(w, prec, dep, sort_every)
         d = r.step(steps, diag=moments)
         res[prec] = r.get_fields()
         dg[prec] = d[-1] if moments else None
@@ -146,50 +156,70 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(w, budget_steps=3):
-    """The reference's own step (oracle/_ref/quill_ref: the reference's
-    src/*.cpp units, single-threaded as the reference runs per rank) on the
-    same workload: full grid, one warm-up step, then `budget_steps` timed
-    steps.  Falls back to the FP64 C restatement when the reference build is
-    absent."""
+def ref_binary():
+    """The reference harness build for this host: the x86-64-v4 (AVX-512)
+    build when the CPU has that feature set, else the x86-64-v3 build."""
+    import oracle
+
+    v4 = oracle.REF_BIN + "_v4"
+    need = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((set(line.split(":", 1)[1].split()) for line in f if line.startswith("flags")), set())
+    except OSError:
+        flags = set()
+    if os.path.exists(v4) and need <= flags:
+        return v4, "x86-64-v4"
+    return oracle.REF_BIN, "x86-64-v3"
+
+
+def cpu_baseline(w, budget_steps=3, grid=None, warm=1):
+    """The reference's own step (oracle/_ref/quill_ref*: the reference's
+    src/*.cpp units, -O3, single-threaded as the reference runs per rank) on
+    the workload's plasma and laser over `grid` (default: the workload's
+    grid): `warm` warm-up steps, then `budget_steps` timed steps.  Falls back
+    to the FP64 C restatement when the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     import golden_io as gio
     import oracle
 
-    nx, ny, nz = w["grid"]
+    nx, ny, nz = grid or w["grid"]
     dx = DX_LAMBDA * TWO_PI
     dt = DT_LAMBDA * TWO_PI
-    lp = laser_params(w, dx)
+    lp = laser_params(dict(w, grid=(nx, ny, nz)), dx)
     L = np.zeros(16)
     L[:8] = [lp["a0y"], lp["a0z"], lp["xsigma"], lp["ysigma"], lp["zsigma"], lp["x0"], lp["b_sign"], lp["x1"]]
     L[11] = 1
     solver = {"fdtd": 0, "ndfx": 1, "five_point": 2}[w["solver"]]
     pusher = {"vay": 0, "boris": 1}[w["pusher"]]
     cores = 1
-    if os.path.exists(oracle.REF_BIN):
+    what = "full grid" if (nx, ny, nz) == tuple(w["grid"]) else \
+        f"x-slab {nx}x{ny}x{nz} of the {'x'.join(map(str, w['grid']))} grid (one GPU's share of an 8-way split)"
+    binary, march = ref_binary()
+    if os.path.exists(binary):
         h = gio.header(mode=gio.MODE_TIME, nx=nx, ny=ny, nz=nz, solver=solver, pusher=pusher, nsteps=budget_steps,
                        seed=SEED & 0x7FFFFFFF, dx=dx, dy=dx, dz=dx, dt=dt, laser=L)
-        h["extra"][:6] = [PPC, w["density"], SIGMA_U, 1, 1, nx]
+        h["extra"][:6] = [PPC, w["density"], SIGMA_U, warm, 1, nx]
         with tempfile.TemporaryDirectory() as td:
             fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
             gio.write_input(fi, h)
-            res = subprocess.run([oracle.REF_BIN, fi, fo], capture_output=True, text=True, check=True)
+            res = subprocess.run([binary, fi, fo], capture_output=True, text=True, check=True)
         rec = json.loads(res.stdout.strip().splitlines()[-1])
         rate = rec["particles"] * rec["steps"] / rec["seconds"]
         return dict(value=rate, unit="particle-steps/s", cores=cores, kind="reference",
-                    sample=f"reference src/*.cpp step (oracle/_ref), full {nx}x{ny}x{nz} grid, "
-                           f"{rec['particles']} particles, 1 warm-up + {rec['steps']} timed steps, "
+                    sample=f"reference src/*.cpp step (oracle/_ref, -O3 -march={march}), {what}, "
+                           f"{rec['particles']} particles, {warm} warm-up + {rec['steps']} timed steps, "
                            f"{rec['seconds']:.2f} s, single thread of {cpu_model()} ({os.cpu_count()} threads)")
     o = oracle.Oracle(nx, ny, nz, dx, dx, dx, dt, solver=solver, pusher=pusher)
     n = o.load_uniform(PPC, w["density"], SIGMA_U, SEED)
     o.laser_cos(*L)
     o.init_boundaries()
-    o.step(1)
+    o.step(warm)
     t0 = time.perf_counter()
     o.step(budget_steps)
     sec = time.perf_counter() - t0
     return dict(value=n * budget_steps / sec, unit="particle-steps/s", cores=cores, kind="port",
-                sample=f"FP64 C restatement (oracle/), full grid, {n} particles, {budget_steps} steps, {sec:.2f} s, "
+                sample=f"FP64 C restatement (oracle/), {what}, {n} particles, {budget_steps} steps, {sec:.2f} s, "
                        f"single thread of {cpu_model()}")
 
 
@@ -377,17 +407,89 @@ def qed_cascade(steps, seed=1, precision="fp32"):
             "growth": [[k, diag[k]["n_e"], diag[k]["n_p"], diag[k]["n_ph"]] for k in range(0, steps, max(1, steps // 8))]}
 
 
+def push_roofline(run, steps, ncell, workload):
+    """Roofline of the fused gather/push/deposit kernel from the HIP events
+    of the particle phase: algorithmic bytes 64 per particle-step (32 B
+    record read and written) + 36 per cell (E and Be gathered, J flushed)."""
+    np_avg = run["pushed"] / steps
+    push_ms = run["ms_particles"] / steps
+    push_bytes = 64.0 * np_avg + 36.0 * ncell
+    achieved = push_bytes / (push_ms * 1e-3) / 1e9
+    traffic = measured_traffic(workload)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": (f"profiles/k_push_tiled_{workload}.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                               "of an earlier run of this workload, copy-calibrated; not measured in this run")
+            if traffic is not None else None,
+            "kernel": "k_push_tiled (gather+push+deposit)", "avg_launch_ms": push_ms,
+            "algorithmic_bytes_per_launch": push_bytes}
+
+
+def workload_block(name, steps, warmup, fp64_steps, err_steps, moments_steps=0):
+    """Throughput, push roofline, plain FP64 method and FP32-vs-FP64 errors
+    of one workload."""
+    w = WORKLOADS[name]
+    nx, ny, nz = w["grid"]
+    ncell = nx * ny * nz
+    se = SORT_EVERY[name]
+    run = timed_run(w, "fp32", "tiled", steps, warmup, se)
+    ms_step = run["seconds"] * 1e3 / steps
+    value = run["pushed"] / run["seconds"]
+    step_bytes = 64.0 * run["pushed"] / steps + 96.0 * ncell
+    out = {
+        "value": value, "unit": "particle-steps/s", "steps": steps, "warmup": warmup, "ms_per_step": ms_step,
+        "config": {"workload": name, "grid": [nx, ny, nz], "cells": ncell, "ppc": PPC, "particles": run["n_start"],
+                   "solver": w["solver"], "pusher": w["pusher"], "qed": False, "density_ncr": w["density"],
+                   "laser_a0": w["a0"], "deposit": "tiled", "sort_every": se, "sort_fallback": SORT_FALLBACK,
+                   "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
+        "roofline": push_roofline(run, steps, ncell, name),
+        "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
+                     "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
+                     "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        # field-only rate: the four field kernels of a step (B half step,
+        # E step, B half step, interpolate_be) over their own time; at FP32
+        # they move 36 + 60 + 36 + 24 = 156 B per cell
+        "field_cell_updates_per_s": ncell * steps / (run["ms_fields"] * 1e-3),
+        "field_phase_GBs": 156.0 * ncell * steps / (run["ms_fields"] * 1e-3) / 1e9,
+        "ms_particles_per_step": run["ms_particles"] / steps,
+        "ms_fields_per_step": run["ms_fields"] / steps,
+        "ms_sort_and_launch_per_step": ms_step - run["ms_events"] / steps,
+        "deposit_fallback_fraction": run["fallback"],
+    }
+    if fp64_steps:
+        plain = timed_run(w, "fp64", "global", fp64_steps, 2, se)
+        v64 = plain["pushed"] / plain["seconds"]
+        out["fp64_plain"] = {"value": v64, "unit": "particle-steps/s", "steps": fp64_steps,
+                             "ms_per_step": plain["seconds"] * 1e3 / fp64_steps, "speedup_fp32": value / v64,
+                             "method": "FP64 SoA, one thread per particle, global FP64 atomics"}
+    if err_steps:
+        out["fp32_vs_fp64_rel_l2"] = field_error(w, err_steps, moments=True, sort_every=se)
+        out["fp32_vs_fp64_steps"] = err_steps
+    if moments_steps:
+        # the whole run length: the components carried by plasma noise
+        # decorrelate (face crossings one step apart change the current),
+        # the laser's components and the global moments do not
+        out[f"fp32_vs_fp64_rel_l2_{moments_steps}_steps"] = field_error(w, moments_steps, moments=True,
+                                                                          sort_every=se)
+    return out
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="uniform-ndfx-vay-256x128x128", choices=sorted(WORKLOADS))
-    ap.add_argument("--fp64-steps", type=int, default=50, help="steps of the plain FP64 comparison run")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default=HEADLINE, choices=sorted(WORKLOADS))
+    ap.add_argument("--fp64-steps", type=int, default=20, help="steps of the plain FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison runs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 256x128x128 and 64x32x32 blocks")
     ap.add_argument("--adjoint", action="store_true",
                     help="only run the adjoint check of the laser-plasma case (128x64x64, 300 steps)")
     ap.add_argument("--adjoint-steps", type=int, default=300)
@@ -395,7 +497,7 @@ def main():
                     help="only run the QED cascade case (128^3, two colliding CP pulses, a0 = 1000)")
     args = ap.parse_args()
     if args.qed_cascade:
-        print(json.dumps(qed_cascade(args.steps if args.steps != 500 else 400)), flush=True)
+        print(json.dumps(qed_cascade(args.steps if args.steps != 20 else 400)), flush=True)
         return
     if args.adjoint:
         print(json.dumps(adjoint_check(args.adjoint_steps, 25)), flush=True)
@@ -407,69 +509,45 @@ def main():
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design, see the sharding notes)", file=sys.stderr)
         sys.exit(2)
-    w = WORKLOADS[args.workload]
-    nx, ny, nz = w["grid"]
-    ncell = nx * ny * nz
-
-    main_run = timed_run(w, "fp32", "tiled", args.steps, args.warmup)
-    K = args.steps
-    ms_step = main_run["seconds"] * 1e3 / K
-    value = main_run["pushed"] / main_run["seconds"]
-    np_avg = main_run["pushed"] / K
-    # dominant kernel: gather + push + deposit, one launch per step (electrons only)
-    push_ms = main_run["ms_particles"] / K
-    push_bytes = 64.0 * np_avg + 36.0 * ncell
-    achieved = push_bytes / (push_ms * 1e-3) / 1e9
-    step_bytes = 64.0 * np_avg + 96.0 * ncell
+    name = args.workload
+    log(f"{name}: {args.steps} steps after {args.warmup} warm-up")
+    head = workload_block(name, args.steps, args.warmup, 0 if args.no_compare else min(args.fp64_steps, args.steps),
+                          0 if args.no_compare else min(50, args.steps))
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "grid": [nx, ny, nz], "cells": ncell, "ppc": PPC,
-                   "particles": main_run["n_start"], "solver": w["solver"], "pusher": w["pusher"], "qed": False,
-                   "deposit": "tiled", "sort_every": SORT_EVERY,
-                   "sort_fallback": SORT_FALLBACK, "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
-                     "kernel": "k_push (gather+push+deposit)", "avg_launch_ms": push_ms,
-                     "algorithmic_bytes_per_launch": push_bytes},
-        "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
-                     "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
-                     "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        # field-only rate: the four field kernels of a step (B half step,
-        # E step, B half step, interpolate_be) over their own time; at FP32
-        # they move 36 + 60 + 36 + 24 = 156 B per cell
-        "field_cell_updates_per_s": ncell * K / (main_run["ms_fields"] * 1e-3),
-        "field_phase_GBs": 156.0 * ncell * K / (main_run["ms_fields"] * 1e-3) / 1e9,
-        "ms_particles_per_step": push_ms,
-        "ms_fields_per_step": main_run["ms_fields"] / K,
-        "ms_sort_and_launch_per_step": ms_step - main_run["ms_events"] / K,
-        "deposit_fallback_fraction": main_run["fallback"],
     }
-    if not args.no_compare:
-        k64 = min(args.fp64_steps, K)
-        plain = timed_run(w, "fp64", "global", k64, 2)
-        v64 = plain["pushed"] / plain["seconds"]
-        line["fp64_plain"] = {"value": v64, "unit": "particle-steps/s", "steps": k64,
-                              "ms_per_step": plain["seconds"] * 1e3 / k64, "speedup_fp32": value / v64,
-                              "method": "FP64 SoA, one thread per particle, global FP64 atomics"}
-        line["fp32_vs_fp64_rel_l2"] = field_error(w, min(k64, 50))
-        line["fp32_vs_fp64_steps"] = min(k64, 50)
-        # the whole run length: the components carried by plasma noise
-        # decorrelate (face crossings one step apart change the current),
-        # the laser's components and the global moments do not
-        line["fp32_vs_fp64_rel_l2_500_steps"] = field_error(w, 500, moments=True)
+    line.update({k: v for k, v in head.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step")})
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w)
+        log("CPU baseline")
+        w = WORKLOADS[name]
+        line["cpu_baseline"] = cpu_baseline(w, budget_steps=2, grid=CPU_SAMPLE_GRID.get(name))
+    if not args.no_extra and name == HEADLINE:
+        # the 256x128x128 NDFX / Vay case: 500 steps, FP32 against FP64
+        # field and moment errors over the whole run
+        c2 = "uniform-ndfx-vay-256x128x128"
+        log(c2)
+        line["uniform_ndfx_vay_256x128x128"] = workload_block(
+            c2, args.steps, args.warmup, 0 if args.no_compare else min(args.fp64_steps, args.steps),
+            0 if args.no_compare else min(50, args.steps), 0 if args.no_compare else 500)
+        # the 64x32x32 FDTD / Boris case: the GPU, and the reference's own
+        # 200 steps on one host core
+        c1 = "uniform-fdtd-boris-64x32x32"
+        log(c1)
+        blk = workload_block(c1, 200, 5, 0, 0)
+        if not args.no_cpu:
+            blk["cpu_reference_200_steps"] = cpu_baseline(WORKLOADS[c1], budget_steps=200, warm=0)
+        line["uniform_fdtd_boris_64x32x32"] = blk
     print(json.dumps(line), flush=True)
 
 

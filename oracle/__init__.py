@@ -81,6 +81,7 @@ def lib():
         L.oq_max_w.argtypes = [vp, C.c_int, C.c_int]
         L.oq_fill_cells.restype = C.c_int64
         L.oq_fill_cells.argtypes = [vp, C.c_double, C.c_int64, ip, dp, ip, C.c_double, C.c_int32]
+        L.oq_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
         L.oq_vacuum_rate.restype = C.c_double
         L.oq_vacuum_rate.argtypes = [C.c_double, dp, dp]
         _lib = L
@@ -93,6 +94,15 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def philox(ctr, key):
+    """Philox4x32-10 of the loader on one block: 4 counter words, 2 key words."""
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib().oq_philox(c, k, out)
+    return list(out)
 
 
 def synth_plasma(nx, ny, nz, ppc, density, sigma_u, seed, ilo=1, ihi=1 << 30):

@@ -1130,6 +1130,13 @@ void oq_init_laser_cos(oq_state *s, double a0y, double a0z, double xsigma, doubl
 /* ------------------------------------------------------------------ */
 /* synthetic uniform plasma from the shared loader (philox.h)          */
 
+void oq_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    oq_u32x4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
+    oq_u32x4 r = oq_philox4x32_10(c, key[0], key[1]);
+    for (int m = 0; m < 4; ++m) out[m] = r.v[m];
+}
+
 int64_t oq_synth_plasma(int nx, int ny, int nz, int ppc, double density_ncr, double sigma_u,
                         uint64_t seed, int ilo, int ihi, double *rec8, int32_t *id)
 {

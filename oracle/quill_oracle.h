@@ -107,6 +107,11 @@ int64_t oq_load_uniform(oq_state *s, int ppc, double density_ncr, double sigma_u
 int64_t oq_synth_plasma(int nx, int ny, int nz, int ppc, double density_ncr, double sigma_u,
                         uint64_t seed, int ilo, int ihi, double *rec8, int32_t *id);
 
+/* Philox4x32-10 of philox.h on one counter block (ctr[4], key[2]) -> out[4],
+ * exposed so the generator can be checked against the published
+ * known-answer vectors */
+void oq_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
 /* single-particle kernels, exposed for closed-form tests */
 void oq_push(int pusher, double cmr, double dt, const double e[3], const double b[3],
              double u[3], double *g);

@@ -60,6 +60,7 @@ struct SpeciesBuf {
     int max_chunks = 0;     // capacity of chunks
     int nchunks = 0;        // host copy after the last sort
     int steps_since_sort = 0;
+    double fb_last = -1;    // window-fallback fraction of the last finished step (-1: unknown or re-sorted since)
     int ishift = 0;         // moving-window shifts since the last sort
     int *tile_start = nullptr;   // ntiles + 1: the tiles of slots [0, n_sorted) (the next sort's source)
 };
@@ -315,6 +316,7 @@ int sort_species(qk_ctx *c, int s)
     b.n_sorted = b.n;
     b.nchunks = c->h_sortinfo[1];
     b.steps_since_sort = 0;
+    b.fb_last = -1;
     b.ishift = 0;
     if (b.nchunks > b.max_chunks) return fail(QK_ERR_STATE, "tile work list overflow");
     return QK_OK;
@@ -766,7 +768,10 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         }
         // The window fallback of step l - 2: once step l - 1 has begun its
         // particle phase, step l - 2 and its diagnostics copy are done, and
-        // the GPU still has step l - 1 queued while the host decides.
+        // the GPU still has step l - 1 queued while the host decides.  The
+        // first two steps of a call use the last step of the previous call
+        // (kept per species), so the rule holds for callers that step one
+        // step per call.
         bool fb_high[QK_MAX_SPECIES] = {};
         if (tiled && c->p.sort_fallback > 0 && l >= 2) {
             HIP_TRY(hipEventSynchronize(c->ev[3 * (l - 1)]));
@@ -775,6 +780,8 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
                 const double live = h[s * kDiagPerSpecies + 2], fb = h[s * kDiagPerSpecies + 3];
                 fb_high[s] = live > 0 && fb > c->p.sort_fallback * live;
             }
+        } else if (tiled && c->p.sort_fallback > 0) {
+            for (int s = 0; s < QK_MAX_SPECIES; ++s) fb_high[s] = c->sp[s].fb_last > c->p.sort_fallback;
         }
         // cell-tile sort: the cadence, a fallback above sort_fallback (seen
         // on a layout at least two steps old), or an unsorted tail (appended
@@ -869,6 +876,15 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         if (diag) unpack_diag(c, h, diag[l]);
     }
     c->fallback_frac = live > 0 ? fb / live : 0;
+    {
+        const double *h = hd + (size_t)(nsteps - 1) * kDiagLen;
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            SpeciesBuf &b = c->sp[s];
+            const double n_s = h[s * kDiagPerSpecies + 2];
+            // a sort after the last push leaves the measurement stale
+            b.fb_last = (n_s > 0 && b.steps_since_sort > 0) ? h[s * kDiagPerSpecies + 3] / n_s : -1;
+        }
+    }
     c->ms_total = t_tot;
     c->ms_part = t_part;
     c->ms_field = t_field;
@@ -933,6 +949,15 @@ int qk_create(const qk_params *params, qk_ctx **out)
     if (p.nx < 5 || p.ny < 5 || p.nz < 5) return fail(QK_ERR_INVALID, "grid must be at least 5 cells per axis");
     if (p.nx > 2048 || p.ny > 1024 || p.nz > 1024)
         return fail(QK_ERR_INVALID, "grid exceeds 2048 x 1024 x 1024 (packed cell index)");
+    {
+        // cell and sort-key indices are 32-bit: the cell count and the tile
+        // keys (whole 8x8x8 tiles) must stay below 2^31
+        const int64_t ncell = (int64_t)p.nx * p.ny * p.nz;
+        const int64_t nkeys = (int64_t)((p.nx + kTx - 1) / kTx) * ((p.ny + kTy - 1) / kTy) *
+                              ((p.nz + kTz - 1) / kTz) * kTileCells;
+        if (ncell >= ((int64_t)1 << 31) || nkeys >= ((int64_t)1 << 31))
+            return fail(QK_ERR_INVALID, "grid too large: cell and tile-key indices must stay below 2^31");
+    }
     if (!(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0)) return fail(QK_ERR_INVALID, "dx, dy, dz, dt must be > 0");
     if (p.solver < 0 || p.solver > 2) return fail(QK_ERR_INVALID, "unknown solver");
     if (p.pusher < 0 || p.pusher > 1) return fail(QK_ERR_INVALID, "unknown pusher");

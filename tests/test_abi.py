@@ -62,6 +62,11 @@ def test_create_rejects_bad_config_without_crashing():
         qa.Region(8, 8, 8, 0.3, 0.3, 0.3, 0.1, solver=7)
     with pytest.raises(qa.QuillError, match="sort_fallback"):
         qa.Region(8, 8, 8, 0.3, 0.3, 0.3, 0.1, sort_fallback=1.5)
+    # cell and tile-key indices are 32-bit: 2048 x 1024 x 1024 cells is 2^31
+    with pytest.raises(qa.QuillError, match="2\\^31"):
+        qa.Region(2048, 1024, 1024, 0.3, 0.3, 0.3, 0.1)
+    with pytest.raises(qa.QuillError, match="2\\^31"):
+        qa.Region(2041, 1017, 1017, 0.3, 0.3, 0.3, 0.1)
 
 
 def test_null_arguments_return_errors():

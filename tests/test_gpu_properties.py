@@ -67,6 +67,23 @@ def test_device_loader_matches_oracle_loader():
     np.testing.assert_array_equal(got[:, 7], rec[:, 6])
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_device_loader_statistics(precision):
+    """qk_load_uniform_plasma at a million particles against the loader's
+    definition: ppc per is_inside cell, uniform 24-bit offsets, N(0, sigma^2)
+    momenta, q = -n / ppc (tests/plasma_checks.py)."""
+    from plasma_checks import check_uniform_plasma
+
+    grid, ppc, density, sigma = (53, 53, 53), 8, 0.01, 0.01
+    print(f"loader statistics {precision} {grid}", flush=True)
+    with qa.Region(*grid, DX, DX, DX, DT, precision=precision) as r:
+        n = r.load_uniform_plasma(qa.ELECTRON, ppc, density, sigma, 20261202)
+        got, _ = r.particles(qa.ELECTRON)
+    assert n == len(got) == 1_000_000
+    check_uniform_plasma(got[:, :3], got[:, 3:6], got[:, 7], grid, ppc, density, sigma,
+                         q_rtol=1e-7 if precision == "fp32" else 0.0)
+
+
 def run_pair(nx, ny, nz, solver, pusher, nsteps, sigma, ppc=2, seed=3):
     """Oracle and a fresh GPU region set up with identical inputs."""
     rng = np.random.default_rng(seed)
@@ -129,18 +146,25 @@ def test_adaptive_sort_matches_oracle():
     o, inputs = run_pair(*shape, "fdtd", "vay", nsteps, sigma=0.6)
     o.step(nsteps)
     frac = {}
-    for fb in (0.0, 0.01):
-        print(f"fp64 tiled, sort_every 0, sort_fallback {fb}", flush=True)
+    # the rule also holds when the steps come one call at a time (a driver
+    # stepping around window shifts and outputs): the fallback of a call's
+    # last step carries over to the next call
+    for fb, calls in ((0.0, 1), (0.01, 1), (0.01, nsteps)):
+        print(f"fp64 tiled, sort_every 0, sort_fallback {fb}, {calls} calls", flush=True)
         r = gpu_region(inputs, shape, "fdtd", "vay", "fp64", "tiled", sort_every=0, sort_fallback=fb)
-        r.step(nsteps)
+        seen = []
+        for _ in range(calls):
+            r.step(nsteps // calls)
+            seen.append(r.fallback_fraction())
         E, B, J, _ = r.get_fields()
-        frac[fb] = r.fallback_fraction()
+        frac[fb, calls] = float(np.mean(seen))
         r.close()
         for name, a, b in (("E", E, o.field(oracle.E)), ("B", B, o.field(oracle.B)), ("J", J, o.field(oracle.J))):
             err = np.abs(a - b).max() / np.abs(b).max()
-            assert err < 1e-10, f"sort_fallback {fb}, {name}: {err:.3e}"
-    assert frac[0.0] > 0.05
-    assert frac[0.01] < 0.5 * frac[0.0]
+            assert err < 1e-10, f"sort_fallback {fb}, {calls} calls, {name}: {err:.3e}"
+    assert frac[0.0, 1] > 0.05
+    assert frac[0.01, 1] < 0.5 * frac[0.0, 1]
+    assert frac[0.01, nsteps] < 0.5 * frac[0.0, 1]
 
 
 def test_fp32_tiled_equals_global_deposit():
