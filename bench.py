@@ -205,10 +205,11 @@ def cpu_baseline(w, budget_steps=3, grid=None, warm=1):
             gio.write_input(fi, h)
             res = subprocess.run([binary, fi, fo], capture_output=True, text=True, check=True)
         rec = json.loads(res.stdout.strip().splitlines()[-1])
-        rate = rec["particles"] * rec["steps"] / rec["seconds"]
+        rate = rec["pushed"] / rec["seconds"]
         return dict(value=rate, unit="particle-steps/s", cores=cores, kind="reference",
                     sample=f"reference src/*.cpp step (oracle/_ref, -O3 -march={march}), {what}, "
-                           f"{rec['particles']} particles, {warm} warm-up + {rec['steps']} timed steps, "
+                           f"{rec['particles']} particles loaded, {warm} warm-up + {rec['steps']} timed steps "
+                           f"pushing {rec['pushed']} particle-steps, "
                            f"{rec['seconds']:.2f} s, single thread of {cpu_model()} ({os.cpu_count()} threads)")
     o = oracle.Oracle(nx, ny, nz, dx, dx, dx, dt, solver=solver, pusher=pusher)
     n = o.load_uniform(PPC, w["density"], SIGMA_U, SEED)

@@ -139,6 +139,19 @@ int qk_get_particles(qk_ctx *ctx, int species, int64_t cap, double *rec10, int32
 /* compute_rho (src/fout.cpp:7-39): node charge of electrons, positrons and
  * photons in the three slots of rho (AoS [nx][ny][nz][3]) */
 int qk_compute_rho(qk_ctx *ctx, double *rho);
+/* A laser entering through an x plane.  No counterpart in the reference,
+ * whose lasers are initial conditions only (f_init_cos, src/init.cpp:6-210,
+ * called from src/main.cpp:1150-1227): here the same pulse (L as in
+ * qk_init_laser_cos, the cos envelope), carried along its direction at c
+ * from where it stands at the next step, is fed in as a total-field /
+ * scattered-field source.  side 0: the total field is held at planes
+ * i >= plane (2 <= plane <= nx - 6), the pulse enters through x-min; side 1:
+ * at i < plane (6 <= plane <= nx - 2), through x-max.  At most 4 sources per
+ * side, sharing one plane.  Each update that reads across the plane is
+ * corrected by the solver's own update applied to the incident field
+ * (sampled on the staggered grid) on the other side. */
+int qk_add_laser_source(qk_ctx *ctx, const double L[16], int side, int plane);
+
 
 /* spatial_region::pmerging (src/mc.cpp:202-462): thin every species once;
  * mode 1 "ti" deletes each particle with probability ppd[0], mode 2 "nl" a

@@ -82,6 +82,9 @@ def lib():
         L.oq_fill_cells.restype = C.c_int64
         L.oq_fill_cells.argtypes = [vp, C.c_double, C.c_int64, ip, dp, ip, C.c_double, C.c_int32]
         L.oq_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.oq_add_laser_source.restype = C.c_int
+        L.oq_add_laser_source.argtypes = [vp, dp, C.c_int, C.c_int]
+        L.oq_laser_field.argtypes = [vp, dp, C.c_double, C.c_int, dp, dp]
         L.oq_vacuum_rate.restype = C.c_double
         L.oq_vacuum_rate.argtypes = [C.c_double, dp, dp]
         _lib = L
@@ -177,6 +180,21 @@ class Oracle:
                   append=True, phi=0.0, xtarget=0.0, ytarget=0.0, ztarget=0.0):
         lib().oq_init_laser_cos(self.h, a0y, a0z, xsigma, ysigma, zsigma, x0, int(b_sign), x1, phase, y0, z0,
                                 int(append), phi, xtarget, ytarget, ztarget)
+
+    def laser_source(self, L, side, plane):
+        """A laser entering through plane `plane` (side 0: total field at
+        i >= plane, side 1: at i < plane); L as laser_cos's 16 arguments."""
+        L = np.ascontiguousarray(L, np.float64)
+        if lib().oq_add_laser_source(self.h, _dp(L), int(side), int(plane)) != 0:
+            raise ValueError("oq_add_laser_source: bad arguments")
+
+    def laser_field(self, L, shift, staggered=True):
+        """(E, B) of the pulse L after travelling `shift` along its direction."""
+        L = np.ascontiguousarray(L, np.float64)
+        E = np.zeros(self.shape + (3,))
+        B = np.zeros(self.shape + (3,))
+        lib().oq_laser_field(self.h, _dp(L), float(shift), int(staggered), _dp(E), _dp(B))
+        return E, B
 
     def pmerging(self, mode, ppd, ion_cmr=()):
         """One merge (mode "ti" or "nl") with deletion fractions ppd[3 + n_ion]."""

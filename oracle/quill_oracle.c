@@ -42,6 +42,17 @@ typedef struct {
     int32_t alive;
 } oq_particle;
 
+/* the pulse of f_init_cos (sscos = 0): its frame and its fields at a point */
+typedef struct {
+    double a0y, a0z, xs, ys, zs;          /* envelope half lengths (init.cpp:34-37) */
+    double x0, y0, z0, xt, yt, zt, phase;
+    double r0[3], ey[3], ez[3];           /* propagation frame (init.cpp:55-93) */
+    double y12, z12;
+    int b_sign;
+} oq_pulse;
+
+#define OQ_MAX_SOURCES 4
+
 struct oq_state {
     oq_params p;
     int64_t ncell;
@@ -49,6 +60,15 @@ struct oq_state {
     oq_particle *part;
     int64_t np, cap;
     uint64_t rng_ctr;          /* sequential Philox stream for QED draws */
+    /* laser sources at an x plane (oq_add_laser_source) */
+    int nsrc;
+    oq_pulse src[OQ_MAX_SOURCES];
+    int src_side[OQ_MAX_SOURCES], src_plane[OQ_MAX_SOURCES];
+    int64_t src_t0[OQ_MAX_SOURCES];
+    int src_shift0[OQ_MAX_SOURCES];
+    int64_t field_steps;       /* oq_maxwell calls so far */
+    int window_shifts;         /* oq_moving_window calls so far */
+    oq_state *tmp;             /* same grid, the sources' correction passes */
 };
 
 /* ------------------------------------------------------------------ */
@@ -111,6 +131,7 @@ oq_state *oq_create(const oq_params *p)
 void oq_destroy(oq_state *s)
 {
     if (!s) return;
+    oq_destroy(s->tmp);
     for (int a = 0; a < 4; ++a) free(s->f[a]);
     free(s->part);
     free(s);
@@ -180,6 +201,7 @@ void oq_moving_window(oq_state *s)
         memset(F + (int64_t)(nx - 1) * plane, 0, (size_t)plane * sizeof(double));
     }
     for (int64_t n = 0; n < s->np; ++n) s->part[n].x -= 1;
+    ++s->window_shifts;
 }
 
 /* get_max_w (low-level.cpp:516-532) */
@@ -602,12 +624,14 @@ static int e_updates(const oq_state *s, int comp, int i, int j, int k)
     return 1;
 }
 
-static void advance_b_half(oq_state *s, const coef6 *cf)
+/* the B half step on planes [ilo, ihi) (every update reads the grid's own
+ * update sets, so a plane range only skips work) */
+static void advance_b_range(oq_state *s, const coef6 *cf, int ilo, int ihi)
 {
-    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const int ny = s->p.ny, nz = s->p.nz;
     const double dtdx = s->p.dt / s->p.dx, dtdy = s->p.dt / s->p.dy, dtdz = s->p.dt / s->p.dz;
     const double *E = s->f[OQ_E];
-    for (int i = 0; i < nx; ++i)
+    for (int i = ilo; i < ihi; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
                 if (b_updates(s, 0, i, j, k))
@@ -619,12 +643,12 @@ static void advance_b_half(oq_state *s, const coef6 *cf)
             }
 }
 
-static void advance_e(oq_state *s, const coef6 *cf)
+static void advance_e_range(oq_state *s, const coef6 *cf, int ilo, int ihi)
 {
-    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const int ny = s->p.ny, nz = s->p.nz;
     const double dtdx = s->p.dt / s->p.dx, dtdy = s->p.dt / s->p.dy, dtdz = s->p.dt / s->p.dz;
     const double *B = s->f[OQ_B];
-    for (int i = 0; i < nx; ++i)
+    for (int i = ilo; i < ihi; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
                 if (e_updates(s, 0, i, j, k))
@@ -636,12 +660,133 @@ static void advance_e(oq_state *s, const coef6 *cf)
             }
 }
 
+static void advance_b_half(oq_state *s, const coef6 *cf) { advance_b_range(s, cf, 0, s->p.nx); }
+static void advance_e(oq_state *s, const coef6 *cf) { advance_e_range(s, cf, 0, s->p.nx); }
+
+static int pulse_frame(const oq_state *s, const double L[16], oq_pulse *P);
+static int pulse_eval(const oq_pulse *P, double px, double py, double pz, double shift, double ev[3], double bv[3]);
+
+/* whether plane i lies on the total-field side of source n */
+static inline int total_side(const oq_state *s, int n, int i)
+{
+    return s->src_side[n] == 0 ? i >= s->src_plane[n] : i < s->src_plane[n];
+}
+
+/*
+ * Total-field / scattered-field laser source.  On the source's total-field
+ * side of its plane the grid holds the total field, on the other side the
+ * field scattered off the incident pulse.  An update that reads across the
+ * plane would mix the two, so every update is corrected by the same update
+ * applied to the incident field on the far side of the plane: with chi = 1 on
+ * the total side, node n of component u gets
+ *     sum_m c_m (chi(n) - chi(n + m)) u_inc(n + m),
+ * i.e. + L(u_inc on the scattered side) on total nodes and
+ * - L(u_inc on the total side) on scattered nodes, L being the solver's own
+ * update (all three solvers, boundary planes included).  The incident field
+ * is the pulse of f_init_cos carried at c, sampled where each component
+ * lives on the staggered grid: E_a at a half cell along a, B_a at half cells
+ * across a.  b_field: correct the B half step (input E_inc at time t), else
+ * the E step (input B_inc at t).
+ */
+static void inject(oq_state *s, const coef6 *cf, int b_field, double t)
+{
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const double d[3] = {s->p.dx, s->p.dy, s->p.dz};
+    oq_state *T = s->tmp;
+    const int in_f = b_field ? OQ_E : OQ_B, out_f = b_field ? OQ_B : OQ_E;
+    const size_t plane_len = (size_t)ny * nz * 3;
+    for (int n = 0; n < s->nsrc; ++n) {
+        const double shift = t - s->src_t0[n] * s->p.dt;
+        /* plane i lies at (i + window shifts since the source was added) dx
+         * in the frame the pulse was given in */
+        const int ish = s->window_shifts - s->src_shift0[n];
+        /* the updates reach two planes: the incident field on the planes
+         * within 4 of the source plane feeds every correction, which is
+         * nonzero within 3 of it; the planes cleared and updated cover what
+         * those corrections read */
+        const int ilo = s->src_plane[n] - 6 > 0 ? s->src_plane[n] - 6 : 0;
+        const int ihi = s->src_plane[n] + 7 < nx ? s->src_plane[n] + 7 : nx;
+        for (int pass = 0; pass < 2; ++pass) {
+            /* pass 0: incident field on the scattered side, read at total
+             * nodes; pass 1: on the total side, read at scattered nodes */
+            memset(T->f[in_f] + ilo * plane_len, 0, (ihi - ilo) * plane_len * sizeof(double));
+            memset(T->f[out_f] + ilo * plane_len, 0, (ihi - ilo) * plane_len * sizeof(double));
+            for (int i = ilo; i < ihi; ++i) {
+                if (abs(i - s->src_plane[n]) > 4 || total_side(s, n, i) != pass) continue;
+                for (int j = 0; j < ny; ++j)
+                    for (int k = 0; k < nz; ++k)
+                        for (int c = 0; c < 3; ++c) {
+                            /* staggered position of component c */
+                            double h[3];
+                            for (int a = 0; a < 3; ++a) h[a] = (b_field ? a == c : a != c) ? 0.5 : 0.0;
+                            double ev[3], bv[3];
+                            if (pulse_eval(&s->src[n], (i + ish + h[0]) * d[0], (j + h[1]) * d[1], (k + h[2]) * d[2],
+                                           shift, ev, bv))
+                                T->f[in_f][IDX(T, i, j, k) + c] = b_field ? ev[c] : bv[c];
+                        }
+            }
+            if (b_field) advance_b_range(T, cf, ilo, ihi);
+            else advance_e_range(T, cf, ilo, ihi);
+            for (int i = ilo; i < ihi; ++i) {
+                if (abs(i - s->src_plane[n]) > 3 || total_side(s, n, i) == pass) continue;
+                const double sg = pass == 0 ? 1.0 : -1.0;
+                for (int j = 0; j < ny; ++j)
+                    for (int k = 0; k < nz; ++k)
+                        for (int c = 0; c < 3; ++c)
+                            s->f[out_f][IDX(s, i, j, k) + c] += sg * T->f[out_f][IDX(T, i, j, k) + c];
+            }
+        }
+    }
+}
+
 void oq_maxwell(oq_state *s)
 {
     coef6 cf = solver_coefs(s);
+    const double t = s->field_steps * s->p.dt;
     advance_b_half(s, &cf);   /* maxwell.cpp:10-22 */
+    if (s->nsrc) inject(s, &cf, 1, t);
     advance_e(s, &cf);
+    if (s->nsrc) inject(s, &cf, 0, t + 0.5 * s->p.dt);
     advance_b_half(s, &cf);
+    if (s->nsrc) inject(s, &cf, 1, t + s->p.dt);
+    ++s->field_steps;
+}
+
+void oq_laser_field(oq_state *s, const double L[16], double shift, int staggered, double *E, double *B)
+{
+    oq_pulse P;
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const double d[3] = {s->p.dx, s->p.dy, s->p.dz};
+    memset(E, 0, (size_t)s->ncell * 3 * sizeof(double));
+    memset(B, 0, (size_t)s->ncell * 3 * sizeof(double));
+    if (pulse_frame(s, L, &P)) return;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k)
+                for (int c = 0; c < 3; ++c)
+                    for (int f = 0; f < 2; ++f) {
+                        double h[3], ev[3], bv[3];
+                        for (int a = 0; a < 3; ++a) h[a] = staggered ? ((f == 0 ? a == c : a != c) ? 0.5 : 0.0) : 0.0;
+                        if (pulse_eval(&P, (i + h[0]) * d[0], (j + h[1]) * d[1], (k + h[2]) * d[2], shift, ev, bv))
+                            (f == 0 ? E : B)[IDX(s, i, j, k) + c] = f == 0 ? ev[c] : bv[c];
+                    }
+}
+
+int oq_add_laser_source(oq_state *s, const double L[16], int side, int plane)
+{
+    if (s->nsrc == OQ_MAX_SOURCES) return -1;
+    if (side == 0 ? !(plane >= 2 && plane <= s->p.nx - 6) : !(side == 1 && plane >= 6 && plane <= s->p.nx - 2))
+        return -1;
+    oq_pulse P;
+    if (pulse_frame(s, L, &P)) return -1;
+    if (!s->tmp) s->tmp = oq_create(&s->p);
+    s->src[s->nsrc] = P;
+    s->src_side[s->nsrc] = side;
+    s->src_plane[s->nsrc] = plane;
+    s->src_t0[s->nsrc] = s->field_steps;
+    s->src_shift0[s->nsrc] = s->window_shifts;
+    ++s->nsrc;
+    return 0;
 }
 
 /*
@@ -1062,22 +1207,33 @@ void oq_step(oq_state *s, int nsteps, oq_diag *diag)
 /* ------------------------------------------------------------------ */
 /* laser IC: f_init_cos, sscos = 0 (init.cpp:6-210)                    */
 
-void oq_init_laser_cos(oq_state *s, double a0y, double a0z, double xsigma, double ysigma,
-                       double zsigma, double x0, int b_sign, double x1, double phase,
-                       double y0, double z0, int append, double phi,
-                       double xtarget, double ytarget, double ztarget)
+
+static int pulse_frame(const oq_state *s, const double L[16], oq_pulse *P)
 {
-    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
-    const double dx = s->p.dx, dy = s->p.dy, dz = s->p.dz;
+    /* L: a0y a0z xsigma ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt */
+    P->a0y = L[0];
+    P->a0z = L[1];
     /* half lengths of the cos^2 envelope with the same \int a^2 as a Gaussian (init.cpp:34-37) */
-    double xs = xsigma * 2 * sqrt(2 * OQ_PI) / 3;
-    double ys = ysigma * 2 * sqrt(2 * OQ_PI) / 3;
-    double zs = zsigma * 2 * sqrt(2 * OQ_PI) / 3;
-    double y12 = 0.5 * ny * dy, z12 = 0.5 * nz * dz;
+    P->xs = L[2] * 2 * sqrt(2 * OQ_PI) / 3;
+    P->ys = L[3] * 2 * sqrt(2 * OQ_PI) / 3;
+    P->zs = L[4] * 2 * sqrt(2 * OQ_PI) / 3;
+    P->x0 = L[5];
+    P->b_sign = L[6] != 0;
+    double x1 = L[7];
+    P->phase = L[8];
+    P->y0 = L[9];
+    P->z0 = L[10];
+    double phi = L[12];
+    P->xt = L[13];
+    P->yt = L[14];
+    P->zt = L[15];
+    P->y12 = 0.5 * s->p.ny * s->p.dy;
+    P->z12 = 0.5 * s->p.nz * s->p.dz;
     /* propagation frame: r0 points from the pulse centre to the target (init.cpp:55-93) */
-    double rn = sqrt(x1 * x1 + y0 * y0 + z0 * z0);
-    double r0[3] = {-x1 / rn, -y0 / rn, -z0 / rn};
-    double ey_[3], ez_[3];
+    double rn = sqrt(x1 * x1 + P->y0 * P->y0 + P->z0 * P->z0);
+    if (!(rn > 0)) return -1;
+    double *r0 = P->r0, *ey_ = P->ey, *ez_ = P->ez;
+    r0[0] = -x1 / rn; r0[1] = -P->y0 / rn; r0[2] = -P->z0 / rn;
     double t = sqrt(r0[0] * r0[0] + r0[1] * r0[1]);
     if (t != 0) { ey_[0] = -r0[1] / t; ey_[1] = r0[0] / t; ey_[2] = 0; }
     else { ey_[0] = 0; ey_[1] = 1; ey_[2] = 0; }
@@ -1089,34 +1245,61 @@ void oq_init_laser_cos(oq_state *s, double a0y, double a0z, double xsigma, doubl
         for (int m = 0; m < 3; ++m) { y[m] = ey_[m]; z[m] = ez_[m]; }
         for (int m = 0; m < 3; ++m) { ey_[m] = y[m] * c + z[m] * sn; ez_[m] = z[m] * c - y[m] * sn; }
     }
+    return 0;
+}
+
+/* Fields of the pulse at the point (px, py, pz) (c/omega units from the
+ * subdomain origin) after it has travelled a distance `shift` along its
+ * direction of propagation: +r0 when b_sign, -r0 otherwise (E x B).  shift = 0
+ * is f_init_cos itself.  Returns 0 (and leaves ev, bv untouched) outside the
+ * envelope. */
+static int pulse_eval(const oq_pulse *P, double px, double py, double pz, double shift, double ev[3], double bv[3])
+{
+    double X = px - P->x0 - P->xt, Y = py - P->y12 - P->y0 - P->yt, Z = pz - P->z12 - P->z0 - P->zt;
+    double x = P->r0[0] * X + P->r0[1] * Y + P->r0[2] * Z;
+    double y = P->ey[0] * X + P->ey[1] * Y + P->ey[2] * Z;
+    double z = P->ez[0] * X + P->ez[1] * Y + P->ez[2] * Z;
+    if (shift != 0) x -= P->b_sign ? shift : -shift;
+    const double xs = P->xs, ys = P->ys, zs = P->zs, a0y = P->a0y, a0z = P->a0z;
+    double xi = x + P->phase;
+    if (!(x < xs && x > -xs && y < ys && y > -ys && z < zs && z > -zs)) return 0;
+    double cx = cos(OQ_PI * x / 2 / xs), cy = cos(OQ_PI * y / 2 / ys), cz = cos(OQ_PI * z / 2 / zs);
+    double sx = sin(OQ_PI * x / 2 / xs), sy = sin(OQ_PI * y / 2 / ys), sz = sin(OQ_PI * z / 2 / zs);
+    double tr = cy * cy * cz * cz;
+    /* transverse fields with the longitudinal-envelope correction
+     * and the divergence-free longitudinal components */
+    double fy = a0y * tr * (cos(xi) * cx * cx - OQ_PI / xs * sin(xi) * cx * sx);
+    double fz = a0z * tr * (sin(xi) * cx * cx + OQ_PI / xs * cos(xi) * cx * sx);
+    double fx = a0y * sin(xi) * cx * cx * cz * cz * OQ_PI / ys * cy * sy
+              - a0z * cos(xi) * cx * cx * cy * cy * OQ_PI / zs * cz * sz;
+    double gx = a0z * cos(xi) * cx * cx * OQ_PI / ys * cz * cz * cy * sy
+              + a0y * sin(xi) * cx * cx * cy * cy * OQ_PI / zs * cz * sz;
+    double gy = -fz, gz = fy;
+    if (!P->b_sign) { gx = -gx; gy = -gy; gz = -gz; }
+    for (int m = 0; m < 3; ++m) {
+        ev[m] = fx * P->r0[m] + fy * P->ey[m] + fz * P->ez[m];
+        bv[m] = gx * P->r0[m] + gy * P->ey[m] + gz * P->ez[m];
+    }
+    return 1;
+}
+
+void oq_init_laser_cos(oq_state *s, double a0y, double a0z, double xsigma, double ysigma,
+                       double zsigma, double x0, int b_sign, double x1, double phase,
+                       double y0, double z0, int append, double phi,
+                       double xtarget, double ytarget, double ztarget)
+{
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const double dx = s->p.dx, dy = s->p.dy, dz = s->p.dz;
+    const double L[16] = {a0y, a0z, xsigma, ysigma, zsigma, x0, b_sign, x1, phase, y0, z0, append, phi,
+                          xtarget, ytarget, ztarget};
+    oq_pulse P;
+    if (pulse_frame(s, L, &P)) return;
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
-                double X = i * dx - x0 - xtarget, Y = j * dy - y12 - y0 - ytarget, Z = k * dz - z12 - z0 - ztarget;
-                double x = r0[0] * X + r0[1] * Y + r0[2] * Z;
-                double y = ey_[0] * X + ey_[1] * Y + ey_[2] * Z;
-                double z = ez_[0] * X + ez_[1] * Y + ez_[2] * Z;
-                double xi = x + phase;
                 double *E = &FE(i, j, k, 0), *B = &FB(i, j, k, 0);
-                if (x < xs && x > -xs && y < ys && y > -ys && z < zs && z > -zs) {
-                    double cx = cos(OQ_PI * x / 2 / xs), cy = cos(OQ_PI * y / 2 / ys), cz = cos(OQ_PI * z / 2 / zs);
-                    double sx = sin(OQ_PI * x / 2 / xs), sy = sin(OQ_PI * y / 2 / ys), sz = sin(OQ_PI * z / 2 / zs);
-                    double tr = cy * cy * cz * cz;
-                    /* transverse fields with the longitudinal-envelope correction
-                     * and the divergence-free longitudinal components */
-                    double fy = a0y * tr * (cos(xi) * cx * cx - OQ_PI / xs * sin(xi) * cx * sx);
-                    double fz = a0z * tr * (sin(xi) * cx * cx + OQ_PI / xs * cos(xi) * cx * sx);
-                    double fx = a0y * sin(xi) * cx * cx * cz * cz * OQ_PI / ys * cy * sy
-                              - a0z * cos(xi) * cx * cx * cy * cy * OQ_PI / zs * cz * sz;
-                    double gx = a0z * cos(xi) * cx * cx * OQ_PI / ys * cz * cz * cy * sy
-                              + a0y * sin(xi) * cx * cx * cy * cy * OQ_PI / zs * cz * sz;
-                    double gy = -fz, gz = fy;
-                    if (!b_sign) { gx = -gx; gy = -gy; gz = -gz; }
-                    double ev[3], bv[3];
-                    for (int m = 0; m < 3; ++m) {
-                        ev[m] = fx * r0[m] + fy * ey_[m] + fz * ez_[m];
-                        bv[m] = gx * r0[m] + gy * ey_[m] + gz * ez_[m];
-                    }
+                double ev[3], bv[3];
+                if (pulse_eval(&P, i * dx, j * dy, k * dz, 0.0, ev, bv)) {
                     for (int m = 0; m < 3; ++m) {
                         if (append) { E[m] += ev[m]; B[m] += bv[m]; }
                         else { E[m] = ev[m]; B[m] = bv[m]; }

@@ -96,6 +96,21 @@ void oq_init_laser_cos(oq_state *s, double a0y, double a0z, double xsigma, doubl
                        double y0, double z0, int append, double phi,
                        double xtarget, double ytarget, double ztarget);
 
+/* A laser entering through an x plane (the reference's lasers are initial
+ * conditions only): the pulse of oq_init_laser_cos's parameters L[16]
+ * (a0y a0z xsigma ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt
+ * zt), carried along its direction at c from its position at the next
+ * oq_maxwell call, is fed in as a total-field / scattered-field source:
+ * side 0 holds the total field at i >= plane (2 <= plane <= nx - 6), side 1
+ * at i < plane (6 <= plane <= nx - 2).  At most 4 sources; returns 0, or -1
+ * on bad arguments. */
+int oq_add_laser_source(oq_state *s, const double L[16], int side, int plane);
+/* the same pulse after travelling `shift` (c/omega units) along its
+ * direction, on the grid of s: staggered = 1 samples each component where it
+ * lives on the staggered grid (E_a half a cell along a, B_a half a cell
+ * across a), 0 at the nodes as f_init_cos does.  E, B: [nx][ny][nz][3]. */
+void oq_laser_field(oq_state *s, const double L[16], double shift, int staggered, double *E, double *B);
+
 /* uniform thermal electron plasma from the shared Philox loader (philox.h);
  * cells with i in [ilo, ihi) (and is_inside); returns particles added */
 int64_t oq_load_uniform(oq_state *s, int ppc, double density_ncr, double sigma_u,

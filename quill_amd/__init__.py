@@ -33,7 +33,7 @@ EXPORTS = ["qk_create", "qk_destroy", "qk_last_error", "qk_set_fields", "qk_get_
            "qk_load_uniform_plasma", "qk_init_laser_cos", "qk_init_boundaries", "qk_step", "qk_sync",
            "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_last_fallback",
            "qk_get_stream", "qk_pmerging", "qk_moving_window", "qk_max_w", "qk_fill_cells", "qk_tape_begin",
-           "qk_adjoint"]
+           "qk_adjoint", "qk_add_laser_source"]
 
 
 class QuillError(RuntimeError):
@@ -95,6 +95,7 @@ def load_library(path=LIB_PATH):
     L.qk_particle_count.argtypes = [vp, C.c_int, i64p]
     L.qk_get_particles.argtypes = [vp, C.c_int, C.c_int64, dp, ip, i64p]
     L.qk_compute_rho.argtypes = [vp, dp]
+    L.qk_add_laser_source.argtypes = [vp, dp, C.c_int, C.c_int]
     L.qk_last_timing.argtypes = [vp, dp, dp, dp]
     L.qk_last_fallback.argtypes = [vp, dp]
     L.qk_get_stream.argtypes = [vp, C.POINTER(vp)]
@@ -205,6 +206,16 @@ class Region:
         L = np.array([a0y, a0z, xsigma, ysigma, zsigma, x0, float(b_sign), x1, phase, y0, z0, float(append), phi,
                       xtarget, ytarget, ztarget], np.float64)
         _check(load_library().qk_init_laser_cos(self._h, _dp(L)), "qk_init_laser_cos")
+
+    def add_laser_source(self, side, plane, a0y, a0z, xsigma, ysigma, zsigma, x0, b_sign=1, x1=0.0, phase=0.0,
+                         y0=0.0, z0=0.0, phi=0.0, xtarget=0.0, ytarget=0.0, ztarget=0.0):
+        """A laser entering through plane `plane` (side 0: x-min, total field at
+        i >= plane; side 1: x-max, total field at i < plane); the pulse is
+        that of init_laser_cos with the same arguments, at the position x0
+        (etc.) when the next step starts."""
+        L = np.array([a0y, a0z, xsigma, ysigma, zsigma, x0, float(b_sign), x1, phase, y0, z0, 1.0, phi,
+                      xtarget, ytarget, ztarget], np.float64)
+        _check(load_library().qk_add_laser_source(self._h, _dp(L), int(side), int(plane)), "qk_add_laser_source")
 
     def init_boundaries(self):
         _check(load_library().qk_init_boundaries(self._h), "qk_init_boundaries")

@@ -80,10 +80,24 @@ struct Tape {
     size_t field_bytes = 0, bytes = 0;
     std::vector<void *> ckpt;     // snapshots at steps 0, every, ...
     std::vector<int> ckpt_jcur;
+    std::vector<uint32_t> ckpt_step;   // step index at each snapshot (laser sources depend on it)
     void *seg = nullptr;          // every per-step states of one segment
     void *fin = nullptr;          // the end state, restored after the adjoint
     double *lam = nullptr;        // 12 field adjoints: E, B, Be, J
     double *lpart = nullptr;      // 7 record adjoints per species
+};
+
+// the laser sources of one side of the box (qk_add_laser_source) and the
+// stack their corrections run on
+struct InjSide {
+    int nsrc = 0;
+    int plane = 0;
+    int W = 0;                    // planes per slab
+    LaserFrame src[kMaxSources];
+    uint32_t t0[kMaxSources] = {};   // step index at which each source was added
+    int shifts0[kMaxSources] = {};   // window shifts at that time
+    void *mem = nullptr;          // 6 field + 6 current arrays of the stack
+    Grid gs{};
 };
 
 }  // namespace
@@ -126,6 +140,9 @@ struct qk_ctx {
     void *d_fill = nullptr;
     size_t fill_cap = 0;
     Tape tape;
+    // laser sources at an x plane, per side (0: x-min, 1: x-max)
+    InjSide inj[2];
+    int window_shifts = 0;          // qk_moving_window calls so far
 };
 
 namespace {
@@ -602,6 +619,7 @@ int tape_snapshot(qk_ctx *c)
     HIP_TRY(hipMalloc(&m, c->tape.bytes));
     c->tape.ckpt.push_back(m);
     c->tape.ckpt_jcur.push_back(c->jcur);
+    c->tape.ckpt_step.push_back(c->step_index);
     return tape_save(c, m);
 }
 
@@ -611,6 +629,7 @@ void tape_free(qk_ctx *c)
     for (void *m : t.ckpt) (void)hipFree(m);
     t.ckpt.clear();
     t.ckpt_jcur.clear();
+    t.ckpt_step.clear();
     if (t.seg) (void)hipFree(t.seg);
     if (t.fin) (void)hipFree(t.fin);
     if (t.lam) (void)hipFree(t.lam);
@@ -682,6 +701,7 @@ int adjoint_impl(qk_ctx *c, const double *lam_E, const double *lam_B, const doub
         const int first = seg * t.every, last = std::min(T, first + t.every);
         QK_TRY(tape_load(c, t.ckpt[seg]));
         c->jcur = t.ckpt_jcur[seg];
+        c->step_index = t.ckpt_step[seg];
         for (int n = first; n < last; ++n) {
             QK_TRY(tape_save(c, (char *)t.seg + (size_t)(n - first) * t.bytes));
             if (n + 1 < last) QK_TRY(step_impl<real>(c, 1, nullptr));
@@ -741,6 +761,74 @@ int adjoint_impl(qk_ctx *c, const double *lam_E, const double *lam_B, const doub
     c->jcur = jcur_end;
     c->step_index = step_end;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+// views of a side's stack: fields (ex..bz; no Be) and two current sets
+template <typename real>
+void stack_views(const InjSide &side, Fields<real> &f, Current<real> &j0, Current<real> &j1)
+{
+    const size_t n = ((size_t)side.gs.ncell * sizeof(real) + 255) & ~(size_t)255;
+    char *m = (char *)side.mem;
+    real *a[12];
+    for (int k = 0; k < 12; ++k) a[k] = (real *)(m + k * n);
+    f.ex = a[0]; f.ey = a[1]; f.ez = a[2];
+    f.bx = a[3]; f.by = a[4]; f.bz = a[5];
+    f.bex = f.bey = f.bez = nullptr;
+    j0.jx = a[6]; j0.jy = a[7]; j0.jz = a[8];
+    j1.jx = a[9]; j1.jy = a[10]; j1.jz = a[11];
+}
+
+// Whether any pulse of the side can be nonzero on its stack planes after
+// travelling shift[n]: the envelope is |x| < xs along the propagation axis,
+// and x is linear in position, so the slab's corner points bound it.
+bool side_active(const qk_ctx *c, const InjSide &side, const InjectArgs &a)
+{
+    const int lo = side.plane - 6 > 0 ? side.plane - 6 : 0;
+    const int hi = side.plane + 6 < c->g.nx ? side.plane + 6 : c->g.nx;
+    for (int n = 0; n < side.nsrc; ++n) {
+        const LaserFrame &L = side.src[n];
+        double xmin = 1e300, xmax = -1e300;
+        for (int corner = 0; corner < 8; ++corner) {
+            const double px = ((corner & 1) ? hi : lo) * c->g.dx + a.ishift[n] * c->g.dx;
+            const double py = ((corner & 2) ? c->g.ny : 0) * c->g.dy, pz = ((corner & 4) ? c->g.nz : 0) * c->g.dz;
+            double x = L.r0[0] * (px - L.x0 - L.xt) + L.r0[1] * (py - L.y12 - L.y0 - L.yt) +
+                       L.r0[2] * (pz - L.z12 - L.z0 - L.zt);
+            x -= L.b_sign ? a.shift[n] : -a.shift[n];
+            xmin = std::min(xmin, x);
+            xmax = std::max(xmax, x);
+        }
+        if (xmax > -L.xs && xmin < L.xs) return true;
+    }
+    return false;
+}
+
+// the laser sources' corrections of one sub-step of fadvance: b_field = 1
+// after a B half step at time (step + frac) dt, 0 after the E step
+template <typename real>
+int inject_impl(qk_ctx *c, int b_field, double frac)
+{
+    for (int sd = 0; sd < 2; ++sd) {
+        InjSide &side = c->inj[sd];
+        if (side.nsrc == 0) continue;
+        InjectArgs a{};
+        a.nsrc = side.nsrc;
+        for (int n = 0; n < side.nsrc; ++n) {
+            a.src[n] = side.src[n];
+            a.shift[n] = ((double)(c->step_index - side.t0[n]) + frac) * c->g.dt;
+            a.ishift[n] = c->window_shifts - side.shifts0[n];
+        }
+        a.side = sd;
+        a.plane = side.plane;
+        a.W = side.W;
+        a.main_lo = sd == 0 ? 0 : c->g.nx - side.W;
+        a.b_field = b_field;
+        if (!side_active(c, side, a)) continue;
+        Fields<real> st;
+        Current<real> j0, j1;
+        stack_views<real>(side, st, j0, j1);
+        HIP_TRY(launch_inject<real>(c->stream, c->p.solver, c->g, c->coef, fields_of<real>(c), side.gs, st, j0, j1, a));
+    }
     return QK_OK;
 }
 
@@ -842,8 +930,11 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         // fields before fadvance (src/main.cpp:2327)
         HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f,
                                        c->d_diag + QK_MAX_SPECIES * kDiagPerSpecies));
+        QK_TRY(inject_impl<real>(c, 1, 0.0));
         HIP_TRY(launch_maxwell_e<real>(c->stream, c->p.solver, c->g, c->coef, f, jc, jn));
+        QK_TRY(inject_impl<real>(c, 0, 0.5));
         HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f, nullptr));
+        QK_TRY(inject_impl<real>(c, 1, 1.0));
         HIP_TRY(launch_interpolate_be<real>(c->stream, c->g, f));
         HIP_TRY(hipEventRecord(c->ev[3 * l + 2], c->stream));
         c->jcur ^= 1;
@@ -1084,6 +1175,8 @@ int qk_destroy(qk_ctx *c)
     if (c->d_maxw) note(hipFree(c->d_maxw));
     if (c->h_maxw) note(hipHostFree(c->h_maxw));
     if (c->d_fill) note(hipFree(c->d_fill));
+    for (auto &side : c->inj)
+        if (side.mem) note(hipFree(side.mem));
     for (auto e : c->ev) note(hipEventDestroy(e));
     if (c->stream) note(hipStreamDestroy(c->stream));
     delete c;
@@ -1317,6 +1410,34 @@ int qk_compute_rho(qk_ctx *c, double *rho)
     return QK_OK;
 }
 
+int qk_add_laser_source(qk_ctx *c, const double L[16], int side, int plane)
+{
+    if (!c || !L) return fail(QK_ERR_INVALID, "null argument");
+    const int nx = c->p.nx;
+    if (side == 0 ? !(plane >= 2 && plane <= nx - 6) : !(side == 1 && plane >= 6 && plane <= nx - 2))
+        return fail(QK_ERR_INVALID, "laser source: side 0 needs 2 <= plane <= nx - 6, side 1 6 <= plane <= nx - 2");
+    InjSide &sd = c->inj[side];
+    if (sd.nsrc == kMaxSources) return fail(QK_ERR_CAPACITY, "at most 4 laser sources per side");
+    if (sd.nsrc > 0 && sd.plane != plane) return fail(QK_ERR_INVALID, "the sources of one side share one plane");
+    LaserFrame F;
+    if (!laser_frame(c->g, L, F)) return fail(QK_ERR_INVALID, "laser source: x1, y0, z0 all zero");
+    if (!sd.mem) {
+        sd.W = side == 0 ? plane + 6 : nx - plane + 6;
+        sd.gs = c->g;
+        sd.gs.nx = 2 * sd.W + kInjGap;
+        sd.gs.ncell = sd.gs.nx * c->g.sx;
+        const size_t n = ((size_t)sd.gs.ncell * c->rsize + 255) & ~(size_t)255;
+        HIP_TRY(hipMalloc(&sd.mem, 12 * n));
+        HIP_TRY(hipMemset(sd.mem, 0, 12 * n));   // the current sets stay zero
+        sd.plane = plane;
+    }
+    sd.src[sd.nsrc] = F;
+    sd.t0[sd.nsrc] = c->step_index;
+    sd.shifts0[sd.nsrc] = c->window_shifts;
+    ++sd.nsrc;
+    return QK_OK;
+}
+
 int qk_pmerging(qk_ctx *c, int mode, const double *ppd)
 {
     if (!c || !ppd) return fail(QK_ERR_INVALID, "null argument");
@@ -1339,6 +1460,7 @@ int qk_moving_window(qk_ctx *c)
         HIP_TRY(launch_shift_particles(c->stream, b.buf[b.cur].cell, b.n));
         ++b.ishift;   // the tiles of the last sort moved with their particles
     }
+    ++c->window_shifts;   // grid plane i now lies at (i + shifts) dx in the frame of the laser sources
     HIP_TRY(hipStreamSynchronize(c->stream));
     return QK_OK;
 }

@@ -281,16 +281,41 @@ __global__ void __launch_bounds__(kBX * kBY) k_init_boundaries(Grid g, int solve
     if (bm & 4) f.bz[id] = real(0);
 }
 
-// Laser initial condition f_init_cos with the cos envelope (src/init.cpp:6-210),
-// evaluated in FP64 and stored in the field precision.  L: a0y a0z xsigma
-// ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt
-struct LaserFrame {
-    double a0y, a0z, xs, ys, zs, x0, phase, xt, yt, zt, y12, z12;
-    double r0[3], ey[3], ez[3];
-    double y0, z0;
-    int b_sign, append;
-};
+// The pulse of f_init_cos with the cos envelope (src/init.cpp:6-210) at the
+// point (px, py, pz) (c/omega units from the subdomain origin), after it has
+// travelled `shift` along its direction of propagation (+r0 when b_sign, -r0
+// otherwise: the direction of E x B); shift = 0 is f_init_cos itself.  FP64.
+// Returns false outside the envelope (ev, bv untouched).
+__device__ inline bool pulse_at(const LaserFrame &L, double px, double py, double pz, double shift, double ev[3],
+                                double bv[3])
+{
+    const double PI = 3.141592653589793;
+    const double X = px - L.x0 - L.xt, Y = py - L.y12 - L.y0 - L.yt, Z = pz - L.z12 - L.z0 - L.zt;
+    double x = L.r0[0] * X + L.r0[1] * Y + L.r0[2] * Z;
+    const double y = L.ey[0] * X + L.ey[1] * Y + L.ey[2] * Z;
+    const double z = L.ez[0] * X + L.ez[1] * Y + L.ez[2] * Z;
+    if (shift != 0) x -= L.b_sign ? shift : -shift;
+    if (!(x < L.xs && x > -L.xs && y < L.ys && y > -L.ys && z < L.zs && z > -L.zs)) return false;
+    const double xi = x + L.phase;
+    const double cx = cos(PI * x / 2 / L.xs), cy = cos(PI * y / 2 / L.ys), cz = cos(PI * z / 2 / L.zs);
+    const double sx = sin(PI * x / 2 / L.xs), sy = sin(PI * y / 2 / L.ys), sz = sin(PI * z / 2 / L.zs);
+    const double sxi = sin(xi), cxi = cos(xi);
+    const double tr = cy * cy * cz * cz;
+    const double fy = L.a0y * tr * (cxi * cx * cx - PI / L.xs * sxi * cx * sx);
+    const double fz = L.a0z * tr * (sxi * cx * cx + PI / L.xs * cxi * cx * sx);
+    const double fx = L.a0y * sxi * cx * cx * cz * cz * PI / L.ys * cy * sy - L.a0z * cxi * cx * cx * cy * cy * PI / L.zs * cz * sz;
+    double gx = L.a0z * cxi * cx * cx * PI / L.ys * cz * cz * cy * sy + L.a0y * sxi * cx * cx * cy * cy * PI / L.zs * cz * sz;
+    double gy = -fz, gz = fy;
+    if (!L.b_sign) { gx = -gx; gy = -gy; gz = -gz; }
+    for (int m = 0; m < 3; ++m) {
+        ev[m] = fx * L.r0[m] + fy * L.ey[m] + fz * L.ez[m];
+        bv[m] = gx * L.r0[m] + gy * L.ey[m] + gz * L.ez[m];
+    }
+    return true;
+}
 
+// Laser initial condition f_init_cos (src/init.cpp:6-210), evaluated in FP64
+// and stored in the field precision
 template <typename real>
 __global__ void __launch_bounds__(kBX * kBY) k_laser_cos(Grid g, Fields<real> f, LaserFrame L)
 {
@@ -298,38 +323,106 @@ __global__ void __launch_bounds__(kBX * kBY) k_laser_cos(Grid g, Fields<real> f,
     const int j = blockIdx.y * kBY + threadIdx.y;
     const int i = blockIdx.z;
     if (k >= g.nz || j >= g.ny) return;
-    const double PI = 3.141592653589793;
-    const double X = i * g.dx - L.x0 - L.xt, Y = j * g.dy - L.y12 - L.y0 - L.yt, Z = k * g.dz - L.z12 - L.z0 - L.zt;
-    const double x = L.r0[0] * X + L.r0[1] * Y + L.r0[2] * Z;
-    const double y = L.ey[0] * X + L.ey[1] * Y + L.ey[2] * Z;
-    const double z = L.ez[0] * X + L.ez[1] * Y + L.ez[2] * Z;
     const int id = i * g.sx + j * g.sy + k;
     real *E[3] = {f.ex, f.ey, f.ez}, *B[3] = {f.bx, f.by, f.bz};
-    if (x < L.xs && x > -L.xs && y < L.ys && y > -L.ys && z < L.zs && z > -L.zs) {
-        const double xi = x + L.phase;
-        const double cx = cos(PI * x / 2 / L.xs), cy = cos(PI * y / 2 / L.ys), cz = cos(PI * z / 2 / L.zs);
-        const double sx = sin(PI * x / 2 / L.xs), sy = sin(PI * y / 2 / L.ys), sz = sin(PI * z / 2 / L.zs);
-        const double sxi = sin(xi), cxi = cos(xi);
-        const double tr = cy * cy * cz * cz;
-        const double fy = L.a0y * tr * (cxi * cx * cx - PI / L.xs * sxi * cx * sx);
-        const double fz = L.a0z * tr * (sxi * cx * cx + PI / L.xs * cxi * cx * sx);
-        const double fx = L.a0y * sxi * cx * cx * cz * cz * PI / L.ys * cy * sy - L.a0z * cxi * cx * cx * cy * cy * PI / L.zs * cz * sz;
-        double gx = L.a0z * cxi * cx * cx * PI / L.ys * cz * cz * cy * sy + L.a0y * sxi * cx * cx * cy * cy * PI / L.zs * cz * sz;
-        double gy = -fz, gz = fy;
-        if (!L.b_sign) { gx = -gx; gy = -gy; gz = -gz; }
+    double ev[3], bv[3];
+    if (pulse_at(L, i * g.dx, j * g.dy, k * g.dz, 0.0, ev, bv)) {
         for (int m = 0; m < 3; ++m) {
-            const double ev = fx * L.r0[m] + fy * L.ey[m] + fz * L.ez[m];
-            const double bv = gx * L.r0[m] + gy * L.ey[m] + gz * L.ez[m];
             if (L.append) {
-                E[m][id] = (real)((double)E[m][id] + ev);
-                B[m][id] = (real)((double)B[m][id] + bv);
+                E[m][id] = (real)((double)E[m][id] + ev[m]);
+                B[m][id] = (real)((double)B[m][id] + bv[m]);
             } else {
-                E[m][id] = (real)ev;
-                B[m][id] = (real)bv;
+                E[m][id] = (real)ev[m];
+                B[m][id] = (real)bv[m];
             }
         }
     } else if (!L.append) {
         for (int m = 0; m < 3; ++m) { E[m][id] = real(0); B[m][id] = real(0); }
+    }
+}
+
+// Total-field / scattered-field laser source (qk_add_laser_source).  On the
+// source's total side of its plane the grid holds the total field, on the
+// other side the field scattered off the incident pulse; an update reading
+// across the plane is corrected by the same update applied to the incident
+// field on the far side: + L(inc on the scattered side) at total nodes,
+// - L(inc on the total side) at scattered nodes, L being the solver's own
+// update (k_maxwell_b / k_maxwell_e) with its boundary-plane sets.  Both
+// passes run as one launch on a stack of two copies of the W planes next to
+// the source's wall, separated by kInjGap empty planes: the copy whose
+// outputs are read at scattered nodes sits at the wall's end of the stack,
+// so its planes near the wall see the grid's own boundary-plane sets.  The
+// incident field is sampled where each component lives on the staggered
+// grid (E_a a half cell along a, B_a a half cell across a).
+
+// the slab of the stack at plane l: 0 low, 1 high, -1 gap; sets the grid plane
+__device__ inline int stack_slab(const InjectArgs &a, int l, int &i)
+{
+    if (l < a.W) { i = a.main_lo + l; return 0; }
+    if (l >= a.W + kInjGap) { i = a.main_lo + l - a.W - kInjGap; return 1; }
+    return -1;
+}
+
+__device__ inline bool total_side(const InjectArgs &a, int i) { return a.side == 0 ? i >= a.plane : i < a.plane; }
+
+// fill the stack's input component (E for a B half step, B for an E step)
+// with the incident field of every source, masked per slab, and clear the
+// output components
+template <typename real>
+__global__ void __launch_bounds__(kBX * kBY) k_inject_prepare(Grid gs, Fields<real> st, InjectArgs a)
+{
+    const int k = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int l = blockIdx.z;
+    if (k >= gs.nz || j >= gs.ny) return;
+    const int id = l * gs.sx + j * gs.sy + k;
+    real *in[3] = {a.b_field ? st.ex : st.bx, a.b_field ? st.ey : st.by, a.b_field ? st.ez : st.bz};
+    real *out[3] = {a.b_field ? st.bx : st.ex, a.b_field ? st.by : st.ey, a.b_field ? st.bz : st.ez};
+    int i = 0;
+    const int slab = stack_slab(a, l, i);
+    // the low slab holds the pass read at scattered nodes when the wall is
+    // at x-min, the pass read at total nodes when it is at x-max
+    const bool reads_at_total = (slab == 1) == (a.side == 0);
+    // pass read at total nodes: incident field on scattered nodes, and back
+    const bool fill = slab >= 0 && abs(i - a.plane) <= 4 && total_side(a, i) != reads_at_total;
+    double v[3] = {0, 0, 0};
+    if (fill) {
+        for (int n = 0; n < a.nsrc; ++n)
+            for (int c = 0; c < 3; ++c) {
+                // staggered position of component c of the input field
+                double h[3];
+                for (int m = 0; m < 3; ++m) h[m] = (a.b_field ? m == c : m != c) ? 0.5 : 0.0;
+                double ev[3], bv[3];
+                if (pulse_at(a.src[n], (i + a.ishift[n] + h[0]) * gs.dx, (j + h[1]) * gs.dy, (k + h[2]) * gs.dz,
+                             a.shift[n], ev, bv))
+                    v[c] += a.b_field ? ev[c] : bv[c];
+            }
+    }
+    for (int c = 0; c < 3; ++c) {
+        in[c][id] = (real)v[c];
+        out[c][id] = real(0);
+    }
+}
+
+// add the stack's corrections to the grid's planes within 3 of the source plane
+template <typename real>
+__global__ void __launch_bounds__(kBX * kBY) k_inject_apply(Grid g, Fields<real> f, Grid gs, Fields<real> st,
+                                                            InjectArgs a)
+{
+    const int k = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int i = a.plane - 3 + (int)blockIdx.z;
+    if (k >= g.nz || j >= g.ny || i < 0 || i >= g.nx) return;
+    const bool tot = total_side(a, i);
+    // the slab read here: at total nodes the high slab for an x-min wall
+    const bool high = tot == (a.side == 0);
+    const int l = (i - a.main_lo) + (high ? a.W + kInjGap : 0);
+    const int id = i * g.sx + j * g.sy + k, ls = l * gs.sx + j * gs.sy + k;
+    real *dst[3] = {a.b_field ? f.bx : f.ex, a.b_field ? f.by : f.ey, a.b_field ? f.bz : f.ez};
+    const real *src[3] = {a.b_field ? st.bx : st.ex, a.b_field ? st.by : st.ey, a.b_field ? st.bz : st.ez};
+    for (int c = 0; c < 3; ++c) {
+        const real d = src[c][ls];
+        dst[c][id] = tot ? dst[c][id] + d : dst[c][id] - d;
     }
 }
 
@@ -403,12 +496,10 @@ hipError_t launch_init_boundaries(hipStream_t st, int solver, const Grid &g, con
     return hipGetLastError();
 }
 
-template <typename real>
-hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f, const double *P)
+bool laser_frame(const Grid &g, const double *P, LaserFrame &L)
 {
     // propagation frame (src/init.cpp:32-93)
     const double PI = 3.141592653589793;
-    LaserFrame L;
     L.a0y = P[0];
     L.a0z = P[1];
     L.xs = P[2] * 2 * sqrt(2 * PI) / 3;
@@ -428,7 +519,7 @@ hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f
     L.y12 = 0.5 * g.ny * g.dy;
     L.z12 = 0.5 * g.nz * g.dz;
     const double rn = sqrt(x1 * x1 + L.y0 * L.y0 + L.z0 * L.z0);
-    if (!(rn > 0)) return hipErrorInvalidValue;
+    if (!(rn > 0)) return false;
     L.r0[0] = -x1 / rn;
     L.r0[1] = -L.y0 / rn;
     L.r0[2] = -L.z0 / rn;
@@ -444,7 +535,31 @@ hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f
         for (int m = 0; m < 3; ++m) { y[m] = L.ey[m]; z[m] = L.ez[m]; }
         for (int m = 0; m < 3; ++m) { L.ey[m] = y[m] * c + z[m] * s; L.ez[m] = z[m] * c - y[m] * s; }
     }
+    return true;
+}
+
+template <typename real>
+hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f, const double *P)
+{
+    LaserFrame L;
+    if (!laser_frame(g, P, L)) return hipErrorInvalidValue;
     k_laser_cos<real><<<cell_grid(g), dim3(kBX, kBY), 0, st>>>(g, f, L);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_inject(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f,
+                         const Grid &gs, const Fields<real> &stack, const Current<real> &j0, const Current<real> &j1,
+                         const InjectArgs &a)
+{
+    k_inject_prepare<real><<<cell_grid(gs), dim3(kBX, kBY), 0, st>>>(gs, stack, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = a.b_field ? launch_maxwell_b<real>(st, solver, gs, c, stack, nullptr)
+                  : launch_maxwell_e<real>(st, solver, gs, c, stack, j0, j1);
+    if (e != hipSuccess) return e;
+    const dim3 grid((g.nz + kBX - 1) / kBX, (g.ny + kBY - 1) / kBY, 7);   // planes plane-3 .. plane+3
+    k_inject_apply<real><<<grid, dim3(kBX, kBY), 0, st>>>(g, f, gs, stack, a);
     return hipGetLastError();
 }
 
@@ -455,7 +570,10 @@ hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f
                                                const Current<real> &, const Current<real> &);                   \
     template hipError_t launch_interpolate_be<real>(hipStream_t, const Grid &, const Fields<real> &);           \
     template hipError_t launch_init_boundaries<real>(hipStream_t, int, const Grid &, const Fields<real> &);     \
-    template hipError_t launch_laser_cos<real>(hipStream_t, const Grid &, const Fields<real> &, const double *);
+    template hipError_t launch_laser_cos<real>(hipStream_t, const Grid &, const Fields<real> &, const double *); \
+    template hipError_t launch_inject<real>(hipStream_t, int, const Grid &, const SolverCoef &, const Fields<real> &, \
+                                            const Grid &, const Fields<real> &, const Current<real> &,           \
+                                            const Current<real> &, const InjectArgs &);
 QK_INST(float)
 QK_INST(double)
 #undef QK_INST

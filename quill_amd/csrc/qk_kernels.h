@@ -161,6 +161,38 @@ template <typename real>
 hipError_t launch_init_boundaries(hipStream_t st, int solver, const Grid &g, const Fields<real> &f);
 template <typename real>
 hipError_t launch_laser_cos(hipStream_t st, const Grid &g, const Fields<real> &f, const double *L);
+
+// the pulse of f_init_cos (src/init.cpp:6-210): envelope half lengths,
+// centre, phase and propagation frame
+struct LaserFrame {
+    double a0y, a0z, xs, ys, zs, x0, phase, xt, yt, zt, y12, z12;
+    double r0[3], ey[3], ez[3];
+    double y0, z0;
+    int b_sign, append;
+};
+// frame of the 16 f_init_cos parameters on grid g; false if degenerate
+bool laser_frame(const Grid &g, const double *P, LaserFrame &L);
+
+// laser sources entering through one x plane: corrections of one sub-step
+constexpr int kMaxSources = 4;
+constexpr int kInjGap = 4;   // empty planes between the two slabs of the stack
+struct InjectArgs {
+    int nsrc;
+    LaserFrame src[kMaxSources];
+    double shift[kMaxSources];   // distance each pulse has travelled at this sub-step
+    int ishift[kMaxSources];     // moving-window shifts since the source was added
+    int side;                    // 0: total field at i >= plane; 1: at i < plane
+    int plane;
+    int W;                       // planes per slab
+    int main_lo;                 // grid plane of slab plane 0
+    int b_field;                 // 1: B half step (input E_inc), 0: E step (input B_inc)
+};
+// the stack grid gs (2 W + kInjGap planes) holds the incident field and the
+// correction passes; j0, j1 zero current buffers of the stack's size
+template <typename real>
+hipError_t launch_inject(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, const Fields<real> &f,
+                         const Grid &gs, const Fields<real> &stack, const Current<real> &j0, const Current<real> &j1,
+                         const InjectArgs &a);
 template <typename real>
 hipError_t launch_load_uniform(hipStream_t st, const Grid &g, const Species<real> &s, int64_t first, int ppc,
                                double q, double sigma, uint64_t seed, int ilo, int ihi);

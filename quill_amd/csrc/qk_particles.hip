@@ -40,6 +40,15 @@ constexpr int kT[3] = {kTx, kTy, kTz};
 #define QK_TATTR
 #endif
 constexpr int kTBlock = QK_TBLOCK;
+// software pipeline of the tiled kernel: gather the next round's field nodes
+// while the current particle is pushed (1) or at use (0); records prefetched
+// one or two rounds ahead
+#ifndef QK_GATHER_AHEAD
+#define QK_GATHER_AHEAD 1
+#endif
+#ifndef QK_RECORDS_AHEAD
+#define QK_RECORDS_AHEAD 2
+#endif
 
 // Current window, FP64, one block per component.  A deposit from start cell
 // r touches cells r-1 .. r+1 (one face crossing per axis at most) and, for
@@ -193,7 +202,7 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
     uint32_t p = (uint32_t)ch.y + threadIdx.x;
     PRec<real> r1, r2;   // the records one and two rounds ahead
     if (p < end32) load_rec32(s, p, r1);
-    if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r2);
+    if (QK_RECORDS_AHEAD > 1 && p + kTBlock < end32) load_rec32(s, p + kTBlock, r2);
     for (int w = threadIdx.x; w < kJWin; w += kTBlock) win[w] = 0.0;
     {
         const int si = i0 - kHalo, sj = j0 - kHalo, sk = k0 - kHalo;
@@ -236,21 +245,38 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
         if (w) load_corners_stage(stage, a, b, cc, nd);
         else load_corners(f, cell_base(g, q.cell), g.sx, g.sy, nd);
     };
+#if QK_GATHER_AHEAD
     int ri1 = 0, rj1 = 0, rk1 = 0;
     bool w1 = false;
     Corners<real> nodes1;
     if (p < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
+#endif
     for (; p < end32; p += kTBlock) {
         PRec<real> r = r1;
+#if QK_GATHER_AHEAD
         Corners<real> nodes = nodes1;
         ri = ri1;
         rj = rj1;
         rk = rk1;
         in_win = w1;
-        r1 = r2;
-        if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
-        if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, r2);
+#endif
+        if (QK_RECORDS_AHEAD > 1) {
+            r1 = r2;
+#if QK_GATHER_AHEAD
+            if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
+#endif
+            if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, r2);
+        } else {
+            if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r1);
+#if QK_GATHER_AHEAD
+            if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
+#endif
+        }
         if (r.cell >= 0) {
+#if !QK_GATHER_AHEAD
+            Corners<real> nodes;
+            gather(r, ri, rj, rk, in_win, nodes);
+#endif
             advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
             store_rec32(s, p, r);
         }

@@ -7,7 +7,7 @@
   component and per moment to the bars stated below.
 * 512 x 256 x 256 cells, FDTD + Boris, 0.001 n_cr at 8 per cell (260.6 M
   electrons), laser a0 = 4, waist 6 lambda, 8 lambda/c, 20 steps with the
-  tile sort of the benchmark: the discrete Gauss law to FP32 round-off,
+  tile sort of the benchmark: the discrete Gauss law to FP32 round-off of E,
   particles lost only through the boundary layers, and FP32 against the
   plain FP64 method on the GPU per field component and moment.
 """
@@ -148,11 +148,15 @@ def test_wakefield_full_size():
             if precision == "fp32":
                 rho = r.compute_rho()
                 g1 = gauss_residual(E, rho)
-                scale = np.abs(rho).max()
-                dg = np.abs(g1 - g0).max() / scale
-                print(f"Gauss law: max |change of div E - rho| = {dg:.2e} max|rho|", flush=True)
-                # FP32 round-off of E over 20 steps; 2e-4 of max|rho| is the bar
-                assert dg < 2e-4
+                dg = np.abs(g1 - g0).max()
+                # charge conservation holds to the FP32 round-off of E: each
+                # step rounds E (|E| up to ~4 here, against |rho| ~ 1e-3 for
+                # 0.001 n_cr), so div E - rho may drift by about
+                # steps x 2^-23 x max|E| / dx; the bar is twice that
+                bar = 2 * steps * 2.0 ** -23 * np.abs(E).max() / DX
+                print(f"Gauss law: max |change of div E - rho| = {dg:.2e} (bar {bar:.2e}; "
+                      f"max|rho| {np.abs(rho).max():.2e}, max|E| {np.abs(E).max():.2e})", flush=True)
+                assert dg < bar
                 del g0, g1, rho
             out[precision] = dict(E=E, B=B, J=J, diag=diag, n=r.count(qa.ELECTRON), n0=n0)
             del E, B, J
