@@ -89,6 +89,8 @@ typedef struct {
     int64_t n_e, n_p, n_ph, n_ion;
     double energy_f, energy_e, energy_p, energy_ph, energy_ion;
     double energy_ions[QK_MAX_ION];   /* per ion population (ienergy, fout.cpp:120-137); energy_ion is their sum */
+    int64_t n_nonfinite;              /* particles dropped this step because their momentum or displacement was
+                                         not finite (corrupt input); the reference would carry the NaN on */
 } qk_diag;
 
 typedef struct qk_ctx qk_ctx;
@@ -192,6 +194,19 @@ int qk_fill_cells(qk_ctx *ctx, int species, int64_t ncells, const int32_t *cells
  * and ids).  No counterpart in the reference: it differentiates the loop
  * body of src/main.cpp:2319-2332 (padvance and fadvance). */
 int qk_tape_begin(qk_ctx *ctx, int every);
+/* Tangent-linear run along the recorded steps (no counterpart in the
+ * reference): right after qk_tape_begin, seed a perturbation of the state
+ * there -- dE, dB (AoS [nx][ny][nz][3], perturbing the fields as given to
+ * qk_set_fields before qk_init_boundaries) and per species dpart[s] (n x
+ * (x y z ux uy uz q), slots of the tape); any pointer may be NULL (zero).
+ * Each following qk_step carries it forward in FP64 along the same branches
+ * (face crossings) as the forward step; qk_tangent_get reads the tangent of
+ * the current state.  With qk_adjoint on the same tape,
+ * <lam, tangent> = <gradient, seed> for any lam and seed: the dot-product
+ * check of the adjoint. */
+int qk_tangent_begin(qk_ctx *ctx, const double *dE, const double *dB, const double *const *dpart);
+int qk_tangent_get(qk_ctx *ctx, double *dE, double *dB, double *const *dpart);
+
 /* Given the adjoints of the state after the recorded steps -- lam_E, lam_B
  * (AoS [nx][ny][nz][3], may be NULL for zero) and per species slot
  * (QK_MAX_SPECIES pointers, each NULL or n x 7 doubles: x y z ux uy uz q of

@@ -33,7 +33,7 @@ EXPORTS = ["qk_create", "qk_destroy", "qk_last_error", "qk_set_fields", "qk_get_
            "qk_load_uniform_plasma", "qk_init_laser_cos", "qk_init_boundaries", "qk_step", "qk_sync",
            "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_last_fallback",
            "qk_get_stream", "qk_pmerging", "qk_moving_window", "qk_max_w", "qk_fill_cells", "qk_tape_begin",
-           "qk_adjoint", "qk_add_laser_source"]
+           "qk_adjoint", "qk_add_laser_source", "qk_tangent_begin", "qk_tangent_get"]
 
 
 class QuillError(RuntimeError):
@@ -59,7 +59,8 @@ class Diag(C.Structure):
     _fields_ = [("N_e", C.c_double), ("N_p", C.c_double), ("N_ph", C.c_double),
                 ("n_e", C.c_int64), ("n_p", C.c_int64), ("n_ph", C.c_int64), ("n_ion", C.c_int64),
                 ("energy_f", C.c_double), ("energy_e", C.c_double), ("energy_p", C.c_double),
-                ("energy_ph", C.c_double), ("energy_ion", C.c_double), ("energy_ions", C.c_double * MAX_ION)]
+                ("energy_ph", C.c_double), ("energy_ion", C.c_double), ("energy_ions", C.c_double * MAX_ION),
+                ("n_nonfinite", C.c_int64)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -105,6 +106,8 @@ def load_library(path=LIB_PATH):
     L.qk_fill_cells.argtypes = [vp, C.c_int, C.c_int64, ip, dp, ip, C.c_double, C.c_int32, i64p]
     L.qk_tape_begin.argtypes = [vp, C.c_int]
     L.qk_adjoint.argtypes = [vp, dp, dp, C.POINTER(dp), dp, dp, C.POINTER(dp)]
+    L.qk_tangent_begin.argtypes = [vp, dp, dp, C.POINTER(dp)]
+    L.qk_tangent_get.argtypes = [vp, dp, dp, C.POINTER(dp)]
     for name in EXPORTS:
         if name not in ("qk_last_error",):
             getattr(L, name).restype = C.c_int
@@ -349,6 +352,41 @@ class Region:
         _check(L.qk_adjoint(self._h, _dp(lE) if lE is not None else None, _dp(lB) if lB is not None else None, lp,
                             _dp(gE), _dp(gB), gp), "qk_adjoint")
         return gE, gB, grads
+
+    def tangent_begin(self, dE=None, dB=None, dparts=None):
+        """Seed the tangent-linear run right after tape_begin: dE, dB
+        (nx, ny, nz, 3) perturb the fields as set before init_boundaries;
+        dparts {species: (n, 7)} the x y z ux uy uz q of the tape's slots."""
+        keep = []
+
+        def arr(a, shape):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64)
+            if a.shape != shape:
+                raise ValueError(f"tangent seed shape {a.shape} != {shape}")
+            keep.append(a)
+            return _dp(a)
+
+        shape = self.shape + (3,)
+        dp_ = (C.POINTER(C.c_double) * MAX_SPECIES)()
+        for s, n in self._tape_n.items():
+            if dparts is not None and s in dparts and n > 0:
+                dp_[s] = arr(dparts[s], (n, 7))
+        _check(load_library().qk_tangent_begin(self._h, arr(dE, shape), arr(dB, shape), dp_), "qk_tangent_begin")
+
+    def tangent_get(self):
+        """(dE, dB, {species: (n, 7)}): the tangent of the current state."""
+        shape = self.shape + (3,)
+        dE, dB = np.zeros(shape), np.zeros(shape)
+        gp = (C.POINTER(C.c_double) * MAX_SPECIES)()
+        out = {}
+        for s, n in self._tape_n.items():
+            if n > 0:
+                out[s] = np.zeros((n, 7))
+                gp[s] = _dp(out[s])
+        _check(load_library().qk_tangent_get(self._h, _dp(dE), _dp(dB), gp), "qk_tangent_get")
+        return dE, dB, out
 
     # -- stepping -----------------------------------------------------------
     def step(self, nsteps=1, diag=False):

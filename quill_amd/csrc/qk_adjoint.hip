@@ -333,7 +333,119 @@ __global__ void __launch_bounds__(kABlock) k_adj_particles(Grid g, Fields<real> 
     }
 }
 
+// ---- tangent-linear step (forward mode) ----------------------------------
+
+// the deposit of a dual current term: its derivative part is the tangent of
+// the current, added to the tangent current (FP64)
+template <typename D>
+struct TangentAcc {
+    double *dj[3];
+    int sx, sy;
+    int base, cur;
+    __device__ inline void set_cell(int wi, int wj, int wk) { cur = base + wi * sx + wj * sy + wk; }
+    __device__ inline void operator()(int comp, int di, int dj_, int dk, const D &v)
+    {
+        const double t = (double)v.d[0];
+        if (t != 0.0) atomicAdd(dj[comp] + cur + di * sx + dj_ * sy + dk, t);
+    }
+};
+
+template <typename real>
+struct TangentParticles {
+    double *dx[3], *du[3], *dq;
+};
+
+// a dual number with value v and tangent t
+template <typename D, typename real>
+__device__ inline D dual_of(real v, double t)
+{
+    D r((double)v);
+    r.v = v;
+    r.d[0] = (real)t;
+    return r;
+}
+
+// Tangent of one particle's push at step n, from the state before it: the
+// record s with its tangent tp (offsets, momentum, charge), the fields E, Be
+// of step n with their tangents df.  The same templated forward code (gather,
+// pusher, displacement, charge-conserving deposit) runs on Dual<1> numbers;
+// branches follow the values, as in the forward step.  tp becomes the
+// tangent after the step; dJ receives the tangent of the particle's current.
+template <typename real, int PUSHER>
+__global__ void __launch_bounds__(kABlock) k_tlm_particles(Grid g, Fields<real> f, Fields<double> df, Species<real> s,
+                                                          int64_t n, StepConst<real> sc, LamField<double> dJ,
+                                                          TangentParticles<real> tp)
+{
+    const int64_t p = (int64_t)blockIdx.x * kABlock + threadIdx.x;
+    if (p >= n) return;
+    const int c = s.cell[p];
+    if (c < 0) return;
+    if (!is_inside(g, cell_i(c), cell_j(c), cell_k(c))) {
+        // erased by p_boundary before the push
+        tp.dx[0][p] = tp.dx[1][p] = tp.dx[2][p] = 0;
+        tp.du[0][p] = tp.du[1][p] = tp.du[2][p] = 0;
+        tp.dq[p] = 0;
+        return;
+    }
+    using D = Dual<1, real>;
+    const int base = cell_base(g, c);
+    Corners<real> nodes;
+    Corners<double> dn;
+    load_corners(f, base, g.sx, g.sy, nodes);
+    load_corners(df, base, g.sx, g.sy, dn);
+    Corners<D> nd;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            nd.e[a][m] = dual_of<D>(nodes.e[a][m], dn.e[a][m]);
+            nd.b[a][m] = dual_of<D>(nodes.b[a][m], dn.b[a][m]);
+        }
+    D o[3] = {dual_of<D>(s.ox[p], tp.dx[0][p]), dual_of<D>(s.oy[p], tp.dx[1][p]), dual_of<D>(s.oz[p], tp.dx[2][p])};
+    D u[3] = {dual_of<D>(s.ux[p], tp.du[0][p]), dual_of<D>(s.uy[p], tp.du[1][p]), dual_of<D>(s.uz[p], tp.du[2][p])};
+    const D q = dual_of<D>(s.q[p], tp.dq[p]);
+    D e[3], b[3];
+    interp_corners(nd, o[0], o[1], o[2], e, b);
+    b[2] += D(sc.ext_bz);
+    D gam = qk_sqrt(D(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    push<PUSHER>(D(sc.cmr_dt), e, b, u, gam);
+    const D a = D(sc.dt) * qk_rcp(gam);
+    const D d[3] = {u[0] * a * D(sc.inv_d[0]), u[1] * a * D(sc.inv_d[1]), u[2] * a * D(sc.inv_d[2])};
+    TangentAcc<D> acc{{dJ.c[0], dJ.c[1], dJ.c[2]}, g.sx, g.sy, base, base};
+    D pos[3] = {o[0], o[1], o[2]};
+    deposit_path<D>(acc, q, pos, d);
+    // the offset after re-binning differs from pos by a whole number of cells
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        tp.dx[m][p] = (double)pos[m].d[0];
+        tp.du[m][p] = (double)u[m].d[0];
+    }
+}
+
 }  // namespace
+
+template <typename real>
+hipError_t launch_tlm_particles(hipStream_t st, const Grid &g, const Fields<real> &f, const Fields<double> &df,
+                                const Species<real> &s, int64_t n, const PushArgs &a, double *const dJ[3],
+                                double *const tpart[7])
+{
+    if (n <= 0) return hipSuccess;
+    const StepConst<real> sc = step_const<real>(g, a);
+    LamField<double> j{{dJ[0], dJ[1], dJ[2]}};
+    TangentParticles<real> tp{{tpart[0], tpart[1], tpart[2]}, {tpart[3], tpart[4], tpart[5]}, tpart[6]};
+    if (a.pusher == 1)
+        k_tlm_particles<real, 1><<<ablocks(n), kABlock, 0, st>>>(g, f, df, s, n, sc, j, tp);
+    else
+        k_tlm_particles<real, 0><<<ablocks(n), kABlock, 0, st>>>(g, f, df, s, n, sc, j, tp);
+    return hipGetLastError();
+}
+
+template hipError_t launch_tlm_particles<float>(hipStream_t, const Grid &, const Fields<float> &,
+                                                const Fields<double> &, const Species<float> &, int64_t,
+                                                const PushArgs &, double *const[3], double *const[7]);
+template hipError_t launch_tlm_particles<double>(hipStream_t, const Grid &, const Fields<double> &,
+                                                 const Fields<double> &, const Species<double> &, int64_t,
+                                                 const PushArgs &, double *const[3], double *const[7]);
 
 hipError_t launch_adj_fields(hipStream_t st, int solver, const Grid &g, const SolverCoef &sc, double *const lam[12])
 {

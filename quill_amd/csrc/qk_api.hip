@@ -85,6 +85,11 @@ struct Tape {
     void *fin = nullptr;          // the end state, restored after the adjoint
     double *lam = nullptr;        // 12 field adjoints: E, B, Be, J
     double *lpart = nullptr;      // 7 record adjoints per species
+    // tangent-linear run along the recorded steps (qk_tangent_begin)
+    bool tangent = false;
+    double *tfield = nullptr;     // 15 FP64 arrays: E, B, Be, J (two buffers)
+    int tjcur = 0;
+    double *tpart = nullptr;      // 7 record tangents per species (x y z ux uy uz q)
 };
 
 // the laser sources of one side of the box (qk_add_laser_source) and the
@@ -148,7 +153,9 @@ struct qk_ctx {
 namespace {
 
 // per-species accumulators, then the field-energy slots
-constexpr size_t kDiagLen = QK_MAX_SPECIES * kDiagPerSpecies + kEnergySlots;
+constexpr size_t kDiagLen = QK_MAX_SPECIES * kDiagPerSpecies + kEnergySlots + 1;
+// the last slot counts particles dropped for a non-finite momentum
+constexpr size_t kNonfiniteSlot = kDiagLen - 1;
 
 template <typename real>
 Fields<real> fields_of(qk_ctx *c)
@@ -362,6 +369,7 @@ void unpack_diag(const qk_ctx *c, const double *h, qk_diag &d)
     double ef_sum = 0;
     for (int m = 0; m < kEnergySlots; ++m) ef_sum += h[QK_MAX_SPECIES * kDiagPerSpecies + m];
     d.energy_f = ef_sum * ef;
+    d.n_nonfinite = (int64_t)h[kNonfiniteSlot];
 }
 
 int ensure_events(qk_ctx *c, size_t n)
@@ -507,6 +515,7 @@ int qed_particle_phase(qk_ctx *c, const Fields<real> &f, const Current<real> &jc
         a.external_bz = c->p.external_bz;
         a.diag = c->d_diag + s * kDiagPerSpecies;
         a.energy_mode = b.energy_mode;
+        a.nonfinite = c->d_diag + kNonfiniteSlot;
         a.first = 0;
         a.n = b.n;
         const Species<real> v = view<real>(b.buf[b.cur]);
@@ -524,6 +533,7 @@ int qed_particle_phase(qk_ctx *c, const Fields<real> &f, const Current<real> &jc
         a.external_bz = c->p.external_bz;
         a.diag = c->d_diag + QK_PHOTON * kDiagPerSpecies;
         a.energy_mode = 1;
+        a.nonfinite = c->d_diag + kNonfiniteSlot;
         a.first = 0;
         a.n = n_ph0;
         HIP_TRY(launch_photon_qed<real>(c->stream, c->g, f, jc, view<real>(bph.buf[bph.cur]), a, qa,
@@ -634,8 +644,12 @@ void tape_free(qk_ctx *c)
     if (t.fin) (void)hipFree(t.fin);
     if (t.lam) (void)hipFree(t.lam);
     if (t.lpart) (void)hipFree(t.lpart);
+    if (t.tfield) (void)hipFree(t.tfield);
+    if (t.tpart) (void)hipFree(t.tpart);
     t.seg = t.fin = nullptr;
     t.lam = t.lpart = nullptr;
+    t.tfield = t.tpart = nullptr;
+    t.tangent = false;
     t.active = t.replay = t.dirty = false;
     t.steps = 0;
 }
@@ -762,6 +776,33 @@ int adjoint_impl(qk_ctx *c, const double *lam_E, const double *lam_B, const doub
     c->step_index = step_end;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return QK_OK;
+}
+
+// the tangent state: FP64 fields (E, B, Be) and the current buffer `which`
+Fields<double> tangent_fields(const qk_ctx *c)
+{
+    const size_t n = (size_t)c->g.ncell;
+    double *m = c->tape.tfield;
+    Fields<double> f;
+    f.ex = m; f.ey = m + n; f.ez = m + 2 * n;
+    f.bx = m + 3 * n; f.by = m + 4 * n; f.bz = m + 5 * n;
+    f.bex = m + 6 * n; f.bey = m + 7 * n; f.bez = m + 8 * n;
+    return f;
+}
+
+Current<double> tangent_current(const qk_ctx *c, int which)
+{
+    const size_t n = (size_t)c->g.ncell;
+    double *m = c->tape.tfield + (9 + 3 * which) * n;
+    return Current<double>{m, m + n, m + 2 * n};
+}
+
+// the 7 record tangents of species s (slots of the tape)
+void tangent_records(const qk_ctx *c, int s, double *out[7])
+{
+    double *m = c->tape.tpart;
+    for (int t = 0; t < s; ++t) m += 7 * c->tape.n[t];
+    for (int a = 0; a < 7; ++a) out[a] = m + (size_t)a * c->tape.n[s];
 }
 
 // views of a side's stack: fields (ex..bz; no Be) and two current sets
@@ -907,7 +948,16 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
             a.external_bz = c->p.external_bz;
             a.diag = c->d_diag + s * kDiagPerSpecies;
             a.energy_mode = b.energy_mode;
+            a.nonfinite = c->d_diag + kNonfiniteSlot;
             const Species<real> v = view<real>(b.buf[b.cur]);
+            if (tp.tangent && !tp.replay && b.kind == 0 && tp.n[s] > 0) {
+                // the tangent of this push, from the state before it
+                double *tr[7];
+                tangent_records(c, s, tr);
+                const Current<double> tj = tangent_current(c, tp.tjcur);
+                double *dj[3] = {tj.jx, tj.jy, tj.jz};
+                HIP_TRY(launch_tlm_particles<real>(c->stream, c->g, f, tangent_fields(c), v, tp.n[s], a, dj, tr));
+            }
             if (tiled && b.kind == 0) {
                 a.chunks = b.chunks;
                 a.nchunks = b.d_nchunks;
@@ -936,6 +986,16 @@ int step_impl(qk_ctx *c, int nsteps, qk_diag *diag)
         HIP_TRY(launch_maxwell_b<real>(c->stream, c->p.solver, c->g, c->coef, f, nullptr));
         QK_TRY(inject_impl<real>(c, 1, 1.0));
         HIP_TRY(launch_interpolate_be<real>(c->stream, c->g, f));
+        if (tp.tangent && !tp.replay) {
+            // the tangent fields advance with the same (linear) field step
+            const Fields<double> tf = tangent_fields(c);
+            const Current<double> tjc = tangent_current(c, tp.tjcur), tjn = tangent_current(c, tp.tjcur ^ 1);
+            HIP_TRY(launch_maxwell_b<double>(c->stream, c->p.solver, c->g, c->coef, tf, nullptr));
+            HIP_TRY(launch_maxwell_e<double>(c->stream, c->p.solver, c->g, c->coef, tf, tjc, tjn));
+            HIP_TRY(launch_maxwell_b<double>(c->stream, c->p.solver, c->g, c->coef, tf, nullptr));
+            HIP_TRY(launch_interpolate_be<double>(c->stream, c->g, tf));
+            tp.tjcur ^= 1;
+        }
         HIP_TRY(hipEventRecord(c->ev[3 * l + 2], c->stream));
         c->jcur ^= 1;
         HIP_TRY(hipMemcpyAsync(hd + (size_t)l * kDiagLen, c->d_diag, kDiagLen * sizeof(double), hipMemcpyDeviceToHost,
@@ -1563,6 +1623,74 @@ int qk_tape_begin(qk_ctx *c, int every)
     t.active = true;
     QK_TRY(tape_snapshot(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return QK_OK;
+}
+
+int qk_tangent_begin(qk_ctx *c, const double *dE, const double *dB, const double *const *dpart)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    Tape &t = c->tape;
+    if (!t.active || t.steps != 0) return fail(QK_ERR_STATE, "qk_tangent_begin goes right after qk_tape_begin");
+    const int64_t nc = c->g.ncell;
+    int64_t np_total = 0;
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) np_total += t.n[s];
+    if (!t.tfield) HIP_TRY(hipMalloc(&t.tfield, (size_t)nc * 15 * sizeof(double)));
+    if (!t.tpart && np_total > 0) HIP_TRY(hipMalloc(&t.tpart, (size_t)np_total * 7 * sizeof(double)));
+    // seeds: fields AoS -> SoA (then the boundary zeroing and interpolate_be
+    // the set-up applied to the fields), records AoS (x y z ux uy uz q) -> SoA
+    {
+        std::vector<double> h((size_t)nc * 15, 0.0);
+        for (int64_t m = 0; m < nc; ++m)
+            for (int a = 0; a < 3; ++a) {
+                if (dE) h[(size_t)a * nc + m] = dE[3 * m + a];
+                if (dB) h[(size_t)(3 + a) * nc + m] = dB[3 * m + a];
+            }
+        HIP_TRY(hipMemcpy(t.tfield, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+            const int64_t n = t.n[s];
+            if (n == 0) continue;
+            std::vector<double> hp((size_t)n * 7, 0.0);
+            if (dpart && dpart[s])
+                for (int64_t p = 0; p < n; ++p)
+                    for (int a = 0; a < 7; ++a) hp[(size_t)a * n + p] = dpart[s][7 * p + a];
+            double *tr[7];
+            tangent_records(c, s, tr);
+            HIP_TRY(hipMemcpy(tr[0], hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    const Fields<double> tf = tangent_fields(c);
+    HIP_TRY(launch_init_boundaries<double>(c->stream, c->p.solver, c->g, tf));
+    HIP_TRY(launch_interpolate_be<double>(c->stream, c->g, tf));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    t.tjcur = 0;
+    t.tangent = true;
+    return QK_OK;
+}
+
+int qk_tangent_get(qk_ctx *c, double *dE, double *dB, double *const *dpart)
+{
+    if (!c) return fail(QK_ERR_INVALID, "null context");
+    Tape &t = c->tape;
+    if (!t.tangent) return fail(QK_ERR_STATE, "no tangent: call qk_tangent_begin after qk_tape_begin");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int64_t nc = c->g.ncell;
+    std::vector<double> h((size_t)nc * 6);
+    HIP_TRY(hipMemcpy(h.data(), t.tfield, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t m = 0; m < nc; ++m)
+        for (int a = 0; a < 3; ++a) {
+            if (dE) dE[3 * m + a] = h[(size_t)a * nc + m];
+            if (dB) dB[3 * m + a] = h[(size_t)(3 + a) * nc + m];
+        }
+    for (int s = 0; s < QK_MAX_SPECIES; ++s) {
+        const int64_t n = t.n[s];
+        if (n == 0 || !dpart || !dpart[s]) continue;
+        std::vector<double> hp((size_t)n * 7);
+        double *tr[7];
+        tangent_records(c, s, tr);
+        HIP_TRY(hipMemcpy(hp.data(), tr[0], hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < n; ++p)
+            for (int a = 0; a < 7; ++a) dpart[s][7 * p + a] = hp[(size_t)a * n + p];
+    }
     return QK_OK;
 }
 

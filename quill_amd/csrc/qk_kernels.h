@@ -33,6 +33,7 @@ struct PushArgs {
     int64_t first, n;        // slots [first, first + n) (global kernel)
     double *diag;            // kDiagPerSpecies doubles, accumulated atomically
     int energy_mode;         // 0: (g-1) q ; 1: g q (photons) ; 2: (g-1) q / cmr (ions)
+    double *nonfinite;       // FP64 count of particles dropped for a non-finite momentum (may be null)
     // tiled kernel: work list of (tile, first slot, count) chunks
     const int4 *chunks;
     const int *nchunks;      // device counter
@@ -136,6 +137,13 @@ hipError_t launch_fill_cells(hipStream_t st, const Species<real> &s, int64_t fir
 hipError_t launch_adj_fields(hipStream_t st, int solver, const Grid &g, const SolverCoef &c, double *const lam[12]);
 // the transpose of interpolate_be alone (for Be of the first step)
 hipError_t launch_adj_interp_be(hipStream_t st, const Grid &g, double *const lam[12]);
+// the tangent of one species' push at the current state s, fields f (E, Be)
+// and their tangents df: tpart (x y z ux uy uz q, FP64) advances one step,
+// dJ (FP64) receives the tangent of the current
+template <typename real>
+hipError_t launch_tlm_particles(hipStream_t st, const Grid &g, const Fields<real> &f, const Fields<double> &df,
+                                const Species<real> &s, int64_t n, const PushArgs &a, double *const dJ[3],
+                                double *const tpart[7]);
 // the adjoint of one species' push from its recorded state s and fields f
 // (E, Be) of the step: lpart = lx[3], lu[3], lq (FP64, per slot) go from after
 // the step to before it; lE and lBe receive the gather's node adjoints

@@ -323,6 +323,7 @@ __global__ void __launch_bounds__(kBlock) k_photon_move(Grid g, Species<real> s,
             }
             const int ni = cell_i(c) + ci[0], nj = cell_j(c) + ci[1], nk = cell_k(c) + ci[2];
             const bool finite = fabs(u[0] * a) < real(1e30) && fabs(u[1] * a) < real(1e30) && fabs(u[2] * a) < real(1e30);
+            if (!finite) count_nonfinite(sc.nonfinite);
             if (finite && is_inside(g, ni, nj, nk)) {
                 s.cell[p] = pack_cell(ni, nj, nk);
                 s.ox[p] = pos[0];

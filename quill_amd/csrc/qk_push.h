@@ -17,7 +17,15 @@ struct StepConst {
     real cmr_dt, ext_bz, inv_d[3], dt;
     double inv_cmr;
     int energy_mode;
+    double *nonfinite;   // count of particles dropped for a non-finite momentum (FP64 sum, may be null)
 };
+
+// a particle whose momentum or displacement stopped being finite (corrupt
+// input) is dropped and counted, so the step reports it
+__device__ inline void count_nonfinite(double *nonfinite)
+{
+    if (nonfinite) atomicAdd(nonfinite, 1.0);
+}
 
 template <typename real>
 StepConst<real> step_const(const Grid &g, const PushArgs &a)
@@ -31,6 +39,7 @@ StepConst<real> step_const(const Grid &g, const PushArgs &a)
     sc.dt = (real)g.dt;
     sc.inv_cmr = a.cmr != 0 ? 1.0 / a.cmr : 0.0;
     sc.energy_mode = a.energy_mode;
+    sc.nonfinite = a.nonfinite;
     return sc;
 }
 
@@ -224,6 +233,7 @@ __device__ inline void advance_particle(const Grid &g, const Corners<real> &node
     r.u[2] = u[2];
     if (!(fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1))) {
         r.cell = -1;
+        count_nonfinite(sc.nonfinite);
         return;
     }
     dep(i, j, k, base, r.q, pos, d, da);

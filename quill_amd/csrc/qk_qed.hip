@@ -9,9 +9,10 @@
 //   k_vacuum_pairs     Schwinger pairs, one draw per cell
 //
 // One thread per particle; current contributions are global atomics.  New
-// particles are appended with an atomic slot counter to the end of their
-// species buffer and are not visited again in the same step, as particles
-// born during padvance are not.  Random numbers come from counter-based
+// particles are appended to the end of their species buffer, one slot
+// counter atomic per wavefront (the lanes that append take consecutive
+// slots in lane order), and are not visited again in the same step, as
+// particles born during padvance are not.  Random numbers come from counter-based
 // Philox streams keyed by (seed, step, slot, stream tag): statistics match
 // the reference, individual events do not (get_rand is a sequential
 // lagged-Fibonacci generator).
@@ -35,13 +36,29 @@ __device__ inline void gather_global(const Fields<real> &f, const Grid &g, int i
     interp_corners(c, pos[0], pos[1], pos[2], e, b);
 }
 
+// A slot of t's buffer for each active lane of the wavefront: the lowest
+// active lane adds the number of active lanes to the counter once, and
+// every lane takes the slot after the active lanes below it (prefix count of
+// the wavefront's ballot).  Called by exactly the lanes that append.
+template <typename real>
+__device__ inline int wave_slot(const AppendTarget<real> &t)
+{
+    const unsigned long long active = __ballot(1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)active) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(t.count, __popcll(active));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(active & ((1ull << lane) - 1ull));
+}
+
 // Append one particle (cell (i, j, k), offsets o) to t; returns false when
 // the buffer is full (the particle is dropped and the overflow flag set)
 template <typename real>
 __device__ inline bool append(const AppendTarget<real> &t, int i, int j, int k, const real o[3], const real u[3],
                               real q, real chi)
 {
-    const int slot = atomicAdd(t.count, 1);
+    const int slot = wave_slot(t);
     if (slot >= t.cap) {
         atomicOr(t.overflow, 1);
         return false;
@@ -63,9 +80,13 @@ __device__ inline bool append(const AppendTarget<real> &t, int i, int j, int k, 
 // pos to its end and re-bin; returns whether the end cell is_inside
 template <typename real>
 __device__ inline bool deposit_move(const Grid &g, const Current<real> &cur, int &i, int &j, int &k, real pos[3],
-                                    const real d[3], real q)
+                                    const real d[3], real q,
+                                    double *nonfinite)
 {
-    if (!(fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1))) return false;
+    if (!(fabs(d[0]) < real(1) && fabs(d[1]) < real(1) && fabs(d[2]) < real(1))) {
+        count_nonfinite(nonfinite);
+        return false;
+    }
     const int base = i * g.sx + j * g.sy + k;
     GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
     deposit_path<real>(acc, q, pos, d);
@@ -110,7 +131,7 @@ __global__ void __launch_bounds__(kBlock) k_push_qed_lepton(Grid g, Fields<real>
         if (!emit) {
             const real a = sc.dt * qk_rcp(gam);
             const real d[3] = {u[0] * a * sc.inv_d[0], u[1] * a * sc.inv_d[1], u[2] * a * sc.inv_d[2]};
-            alive = deposit_move(g, cur, i, j, k, pos, d, q);
+            alive = deposit_move(g, cur, i, j, k, pos, d, q, sc.nonfinite);
         } else {
             // photon emission (src/advance.cpp:291-357)
             const real a = real(0.5) * sc.dt / g_prev;
@@ -143,7 +164,7 @@ __global__ void __launch_bounds__(kBlock) k_push_qed_lepton(Grid g, Fields<real>
                 push<PUSHER>(sc.cmr_dt, e, b, u, gam);
                 const real a4 = sc.dt * qk_rcp(gam);
                 const real d[3] = {u[0] * a4 * sc.inv_d[0], u[1] * a4 * sc.inv_d[1], u[2] * a4 * sc.inv_d[2]};
-                alive = deposit_move(g, cur, i, j, k, pos, d, q);
+                alive = deposit_move(g, cur, i, j, k, pos, d, q, sc.nonfinite);
             }
         }
         if (alive) {
@@ -186,7 +207,7 @@ __device__ inline void pair_lepton(const Grid &g, const Current<real> &cur, cons
     real pos[3] = {pos0[0], pos0[1], pos0[2]};
     const real a = sc.dt * qk_rcp(gam);
     const real d[3] = {u[0] * a * sc.inv_d[0], u[1] * a * sc.inv_d[1], u[2] * a * sc.inv_d[2]};
-    if (deposit_move(g, cur, i, j, k, pos, d, qb) && append(t, i, j, k, pos, u, qb, chi)) {
+    if (deposit_move(g, cur, i, j, k, pos, d, qb, sc.nonfinite) && append(t, i, j, k, pos, u, qb, chi)) {
         da.q += (double)qb;
         da.n += 1;
         da.e += kinetic(u, gam) * (double)qb;
@@ -227,6 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_photon_qed(Grid g, Fields<real> f, C
             const real a = sc.dt / gam;
             for (int m = 0; m < 3; ++m) pos[m] += u[m] * a * sc.inv_d[m];
             const bool finite = fabs(u[0] * a) < real(1e30) && fabs(u[1] * a) < real(1e30) && fabs(u[2] * a) < real(1e30);
+            if (!finite) count_nonfinite(sc.nonfinite);
             rebin(i, j, k, pos);
             if (finite && is_inside(g, i, j, k)) {
                 s.cell[p] = pack_cell(i, j, k);

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <algorithm>
 #include <vector>
 
 namespace qkd {
@@ -114,6 +115,106 @@ std::string iteration_suffix(double t)
     char buf[100];
     std::snprintf(buf, sizeof buf, "%g", value);
     return buf;
+}
+
+namespace {
+
+constexpr double kPiOut = 3.14159265358979323846;
+constexpr double kProtonMassOut = 1836.1526721;   // include/physical_constants.h:4
+
+// %g of the charge-to-mass ratio, as the ion file names use it
+std::string cmr_name(double cmr)
+{
+    char b[32];
+    std::snprintf(b, sizeof b, "%g", cmr);
+    return b;
+}
+
+}  // namespace
+
+void write_spectrum_phasespace(const std::string &dir, const std::string &suffix, const SpectraConf &c,
+                               const SpeciesRecords *species, int n_ion, const double *icmr, double dx, double dy,
+                               double dz, double lambda_cm, PhasespaceCounters &k)
+{
+    // every stream is opened, written or not, as the reference opens them
+    std::ofstream ps(dir + "/phasespace" + suffix), ps_p(dir + "/phasespace_p" + suffix),
+        ps_ph(dir + "/phasespace_ph" + suffix);
+    std::vector<std::ofstream> ps_i(n_ion);
+    for (int m = 0; m < n_ion; ++m) ps_i[m].open(dir + "/phasespace_" + cmr_name(icmr[m]) + "_" + suffix);
+    std::vector<double> sp(c.neps, 0.0), sp_p(c.neps_p, 0.0), sp_ph(c.neps_ph, 0.0);
+    std::vector<std::vector<double>> sp_i(n_ion, std::vector<double>(c.neps_i, 0.0));
+    auto record = [&](std::ofstream &f, const double *r) {
+        f << r[7] << "\n";
+        f << dx * r[0] / (2 * kPiOut) << "\n";
+        f << dy * r[1] / (2 * kPiOut) << "\n" << dz * r[2] / (2 * kPiOut) << "\n";
+        f << r[3] << "\n" << r[4] << "\n" << r[5] << "\n";
+        f << r[6] << "\n";
+        f << r[9] << "\n";
+    };
+    // the traversal order: cells i, j, k (the reference walks cp[i][j][k]),
+    // the records of one cell in the order given
+    const int ns = 3 + n_ion;
+    struct Ref {
+        int64_t cell;
+        int s;
+        int64_t n;
+    };
+    std::vector<Ref> order;
+    for (int s = 0; s < ns; ++s)
+        for (int64_t n = 0; n < species[s].n; ++n) {
+            const double *r = species[s].rec10 + 10 * n;
+            const int64_t ci = (int64_t)std::floor(r[0]), cj = (int64_t)std::floor(r[1]), ck = (int64_t)std::floor(r[2]);
+            order.push_back(Ref{(ci << 40) | (cj << 20) | ck, s, n});
+        }
+    std::stable_sort(order.begin(), order.end(), [](const Ref &a, const Ref &b) { return a.cell < b.cell; });
+    for (const Ref &o : order) {
+        const double *r = species[o.s].rec10 + 10 * o.n;
+        const double g = r[6], q = r[7];
+        int i_eps;
+        if (o.s == 0) {   // electrons
+            i_eps = (int)((g - 1) * 0.511 / c.deps);
+            if (i_eps > -1 && i_eps < c.neps) sp[i_eps] -= q;   // q < 0 for electrons
+            if (++k.e > c.enthp) {
+                k.e = 0;
+                record(ps, r);
+            }
+        } else if (o.s == 1 && c.write_p) {
+            i_eps = (int)((g - 1) * 0.511 / c.deps_p);
+            if (i_eps > -1 && i_eps < c.neps_p) sp_p[i_eps] += q;
+            if (++k.p > c.enthp_p) {
+                k.p = 0;
+                record(ps_p, r);
+            }
+        } else if (o.s == 2 && c.write_ph) {
+            i_eps = (int)(g * 0.511 / c.deps_ph);
+            if (i_eps > -1 && i_eps < c.neps_ph) sp_ph[i_eps] += q;
+            if (++k.ph > c.enthp_ph) {
+                k.ph = 0;
+                record(ps_ph, r);
+            }
+        } else if (o.s >= 3) {
+            const int m = o.s - 3;
+            i_eps = (int)((g - 1) * 0.511 * kProtonMassOut / c.deps_i);
+            if (i_eps > -1 && i_eps < c.neps_i) sp_i[m][i_eps] += q;
+            if (++k.i > c.enthp_i) {
+                k.i = 0;
+                record(ps_i[m], r);
+            }
+        }
+    }
+    // dN/deps in 1/MeV (nucleons per MeV for ions)
+    const double base = 1.11485e13 * lambda_cm * dx * dy * dz / (8 * kPiOut * kPiOut * kPiOut);
+    std::ofstream fs(dir + "/spectrum" + suffix), fs_p(dir + "/spectrum_p" + suffix),
+        fs_ph(dir + "/spectrum_ph" + suffix);
+    std::vector<std::ofstream> fs_i(n_ion);
+    for (int m = 0; m < n_ion; ++m) fs_i[m].open(dir + "/spectrum_" + cmr_name(icmr[m]) + "_" + suffix);
+    for (double v : sp) fs << v * (base / c.deps) << "\n";
+    if (c.write_p)
+        for (double v : sp_p) fs_p << v * (base / c.deps_p) << "\n";
+    if (c.write_ph)
+        for (double v : sp_ph) fs_ph << v * (base / c.deps_ph) << "\n";
+    for (int m = 0; m < n_ion; ++m)
+        for (double v : sp_i[m]) fs_i[m] << v * (base / (icmr[m] * c.deps_i)) << "\n";
 }
 
 }  // namespace qkd

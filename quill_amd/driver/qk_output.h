@@ -12,6 +12,7 @@
 // always text.
 #pragma once
 
+#include <cstdint>
 #include <string>
 
 namespace qkd {
@@ -43,6 +44,39 @@ struct OutputSet {
 void write_output_set(const std::string &dir, const std::string &suffix, const OutputGrid &g, const OutputSet &s,
                       const double *E, const double *B, const double *Be, const double *rho, const double *J,
                       double j_scale);
+
+// Spectra and phase space (write_spectrum_phasespace, src/main.cpp:478-743)
+struct SpectraConf {
+    double deps = 0, deps_p = 0, deps_ph = 0, deps_i = 0;   // bin widths, MeV (MeV per nucleon for ions)
+    int neps = 0, neps_p = 0, neps_ph = 0, neps_i = 0;       // bins
+    int enthp = 0, enthp_p = 0, enthp_ph = 0, enthp_i = 0;   // every (enthp + 1)-th particle goes to phasespace
+    bool write_p = false, write_ph = false;                  // particles_for_output has p / ph
+    std::string particles_for_output = "e";
+};
+
+// the reference's running counters of the phase-space thinning (global in
+// the reference, src/main.cpp:66, so they carry over between outputs)
+struct PhasespaceCounters {
+    int e = 0, p = 0, ph = 0, i = 0;
+};
+
+// the live particles of one species as qk_get_particles gives them:
+// n x (x y z ux uy uz g q cmr chi), positions in cells
+struct SpeciesRecords {
+    const double *rec10 = nullptr;
+    int64_t n = 0;
+};
+
+// spectrum<suffix>, spectrum_p, spectrum_ph, spectrum_<cmr>_ and
+// phasespace<suffix>, phasespace_p, phasespace_ph, phasespace_<cmr>_ for one
+// subdomain (n_sr = 1): every particle in the reference's traversal order
+// (cells i, j, k; within a cell the order of the records given), binned by
+// energy and thinned to every (enthp + 1)-th into 9-line records
+// (q, x y z in lambda, ux uy uz, g, chi).  species: electrons, positrons,
+// photons, then the n_ion ion populations of charge-to-mass ratios icmr.
+void write_spectrum_phasespace(const std::string &dir, const std::string &suffix, const SpectraConf &c,
+                               const SpeciesRecords *species, int n_ion, const double *icmr, double dx, double dy,
+                               double dz, double lambda_cm, PhasespaceCounters &k);
 
 // file-name suffix of an output time t (c/omega units):
 // OutputIterations::get_current_iteration_string (src/parameters.cpp:42-50)

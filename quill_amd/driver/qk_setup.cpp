@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <sstream>
 
 namespace qkd {
 
@@ -160,6 +161,8 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
             period.push_back(op);
         } while (c.get("t_end", from).value != 0);
         output_iterations(t_end, period, dt, run.out_iter, run.out_time);
+        run.t_end_list = t_end;
+        run.period_list = period;
         run.last_iteration = run.out_iter.back();
     }
 
@@ -361,6 +364,13 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
                 err = "thermal films (Tfilm != 0) are not supported";
                 return false;
             }
+            // as the reference's film object keeps them, for the log
+            f.log_width = len("filmwidth", true);
+            f.log_y1 = len("y1film", true);
+            f.log_z1 = len("z1film", true);
+            f.log_ne = nf;
+            f.log_mcr = mcr;
+            for (int a = 0; a < 3; ++a) f.log_npic[a] = f.npic[a];
             films.push_back(f);
             run.ion_mcr.push_back(mcr);
         }
@@ -413,8 +423,40 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
         const std::string pf = c.get("particles_for_output").units;
         run.out_set.rho_p = pf == "ep" || pf == "epph";
         run.out_set.rho_ph = pf == "eph" || pf == "epph";
+        // spectra and phase space (src/main.cpp:3337-3395): bin widths in MeV
+        // (given in MeV or mc^2), 0 meaning the electrons' value
+        SpectraConf &sc = run.spectra;
+        auto mev = [&](const char *name) {
+            const ConfVar &v = c.get(name);
+            return v.units == "mc^2" ? v.value * 0.511 : v.value;
+        };
+        sc.deps = mev("deps");
+        sc.deps_p = mev("deps_p");
+        if (sc.deps_p == 0) sc.deps_p = sc.deps;
+        sc.deps_ph = mev("deps_ph");
+        if (sc.deps_ph == 0) sc.deps_ph = sc.deps;
+        sc.deps_i = c.get("deps_i").value;   // MeV per nucleon
+        if (sc.deps_i == 0) sc.deps_i = sc.deps;
+        sc.neps = (int)c.get("neps").value;
+        sc.neps_p = (int)c.get("neps_p").value;
+        if (sc.neps_p == 0) sc.neps_p = sc.neps;
+        sc.neps_ph = (int)c.get("neps_ph").value;
+        if (sc.neps_ph == 0) sc.neps_ph = sc.neps;
+        sc.neps_i = (int)c.get("neps_i").value;
+        if (sc.neps_i == 0) sc.neps_i = sc.neps;
+        sc.enthp = (int)c.get("enthp").value;
+        sc.enthp_p = (int)c.get("enthp_p").value;
+        if (sc.enthp_p == 0) sc.enthp_p = sc.enthp;
+        sc.enthp_ph = (int)c.get("enthp_ph").value;
+        if (sc.enthp_ph == 0) sc.enthp_ph = sc.enthp;
+        sc.enthp_i = (int)c.get("enthp_i").value;
+        if (sc.enthp_i == 0) sc.enthp_i = sc.enthp;
+        sc.particles_for_output = pf.empty() ? "e" : pf;
+        sc.write_p = run.out_set.rho_p;
+        sc.write_ph = run.out_set.rho_ph;
         double x0fout = length(c, c.get("x0fout"), true);
         if (x0fout == 0) x0fout = xlength / 2;
+        run.x0fout = x0fout;
         run.out_grid.i_yz = (int)((xlength - x0fout) / dx);
         run.out_grid.binary = c.get("output_mode").units == "binary";
     }
@@ -529,6 +571,165 @@ bool build_run(const std::vector<ConfVar> &vars, Run &run, std::string &err)
                 mirror(F[0], F[0], -1, +1, +1);
             }
         }
+    }
+    // the log file main() writes after init() (src/main.cpp:3611-3787):
+    // key and value lines up to "$", then the run switches and a summary;
+    // lengths and times in lambda units, densities in cm^-3
+    {
+        std::ostringstream L;
+        const double tp = 2 * kPi;
+        auto kv = [&](const char *k, auto v) { L << k << "\n" << v << "\n"; };
+        auto str = [&](const char *name, const char *dflt) {
+            const std::string &u = c.get(name).units;
+            return u.empty() ? std::string(dflt) : u;
+        };
+        kv("dx", dx / 2 / kPi);
+        kv("dy", dy / 2 / kPi);
+        kv("dz", dz / 2 / kPi);
+        kv("dt", dt / 2 / kPi);
+        kv("nx", int(xlength / dx));
+        kv("ny", int(ylength / dy));
+        kv("nz", int(zlength / dz));
+        kv("lambda", c.lambda);
+        kv("ne", ne);
+        for (int i = (int)run.t_end_list.size() - 1; i >= 0; i--) {
+            kv("t_end", run.t_end_list[i] / 2 / kPi);
+            kv("output_period", run.period_list[i] / 2 / kPi);
+        }
+        L << "t_add_mw\n" << run.t_add_mw / tp << '\n';
+        kv("xsigma", xsigma / 2 / kPi);
+        kv("ysigma", ysigma / 2 / kPi);
+        kv("zsigma", zsigma / 2 / kPi);
+        kv("x0fout", run.x0fout / 2 / kPi);
+        kv("a0y", a0y);
+        kv("a0z", a0z);
+        kv("external_bz", p.external_bz);
+        kv("mwindow", run.mwindow == 0 ? "off" : (run.mwindow == 1 ? "on" : "auto"));
+        if (run.mwindow != 0) {
+            kv("mwseed", run.mwseed ? "on" : "off");
+            L << "mwseed_ions\n" << (run.mwseed_ions ? "on" : str("mwseed_ions", "off")) << '\n';
+            if (run.mwseed_ions) {
+                double mw_mcr = c.get("mw_mcr").value;
+                L << "mw_mcr\n" << (mw_mcr == 0 ? 1.0 : mw_mcr) << '\n';
+            }
+        }
+        kv("e_components_for_output", str("e_components_for_output", "none"));
+        kv("b_components_for_output", str("b_components_for_output", "none"));
+        kv("f_envelope", f_envelope);
+        kv("b_sign", 2.0 * b_sign - 1);
+        kv("x0", x0 / 2 / kPi);
+        kv("y0", y00 / 2 / kPi);
+        kv("z0", z00 / 2 / kPi);
+        kv("xtarget", target[0] / 2 / kPi);
+        kv("ytarget", target[1] / 2 / kPi);
+        kv("ztarget", target[2] / 2 / kPi);
+        kv("phase", phase);
+        kv("lp_reflection", str("lp_reflection", "off"));
+        kv("f_reflection", str("f_reflection", "off"));
+        kv("phi", phi);
+        kv("phi_rotate", phi_rotate);
+        kv("shenergy", c.get("shenergy").value);
+        kv("shphase", c.get("shphase").units == "pi" ? c.get("shphase").value * kPi : c.get("shphase").value);
+        kv("beam", str("beam", "off"));
+        kv("beam_particles", str("beam_particles", "e"));
+        kv("Nb", c.get("Nb").value);
+        kv("epsb", c.get("epsb").units == "mc^2" ? c.get("epsb").value * 0.511 : c.get("epsb").value);
+        kv("xb", length(c, c.get("xb"), true) / 2 / kPi);
+        kv("rb", length(c, c.get("rb"), false) / 2 / kPi);
+        kv("x0b", length(c, c.get("x0b"), true) / 2 / kPi);
+        kv("y0b", length(c, c.get("y0b"), false) / 2 / kPi);
+        kv("phib", c.get("phib").value);
+        kv("mwspeed", run.mwspeed);
+        kv("nelflow", 0.0);
+        kv("vlflow", c.get("vlflow").value);
+        kv("mcrlflow", c.get("mcrlflow").value);
+        kv("Tlflow", c.get("Tlflow").value);
+        kv("nerflow", 0.0);
+        kv("vrflow", c.get("vrflow").value);
+        kv("mcrrflow", c.get("mcrrflow").value);
+        kv("Trflow", c.get("Trflow").value);
+        kv("ions", ions);
+        for (const Film &f : run.films) {   // newest first, as p_last_film -> prev
+            kv("film", "on");
+            kv("x0film", f.x0 / 2 / kPi);
+            kv("filmwidth", f.log_width / 2 / kPi);
+            kv("gradwidth", f.gradwidth / 2 / kPi);
+            kv("gradwidth_y", f.gradwidth_y / 2 / kPi);
+            kv("y0film", f.y0 / 2 / kPi);
+            kv("y1film", f.log_y1 / 2 / kPi);
+            kv("z0film", f.z0 / 2 / kPi);
+            kv("z1film", f.log_z1 / 2 / kPi);
+            kv("nfilm", f.log_ne);
+            kv("mcr", f.log_mcr);
+            kv("Tfilm", f.T);
+            kv("vxfilm", f.vx);
+            kv("xnpic_film", f.log_npic[0]);
+            kv("ynpic_film", f.log_npic[1]);
+            kv("znpic_film", f.log_npic[2]);
+        }
+        kv("n_ion_populations", p.n_ion);
+        for (int i = 0; i < p.n_ion; i++) kv("icmr", p.ion_cmr[i]);
+        kv("xnpic", xnpic);
+        kv("ynpic", ynpic);
+        kv("znpic", znpic);
+        const SpectraConf &sc = run.spectra;
+        kv("particles_for_output", sc.particles_for_output);
+        kv("deps", sc.deps);
+        kv("neps", sc.neps);
+        kv("enthp", sc.enthp);
+        kv("deps_p", sc.deps_p);
+        kv("neps_p", sc.neps_p);
+        kv("enthp_p", sc.enthp_p);
+        kv("deps_ph", sc.deps_ph);
+        kv("neps_ph", sc.neps_ph);
+        kv("enthp_ph", sc.enthp_ph);
+        kv("deps_i", sc.deps_i);
+        kv("neps_i", sc.neps_i);
+        kv("enthp_i", sc.enthp_i);
+        kv("n_sr", 1);
+        kv("n_tracks", (int)c.get("n_tracks").value);
+        std::string ptt = str("particles_to_track", "epgi");
+        if (ptt == "none") ptt = "";
+        kv("particles_to_track", ptt);
+        {   // src/main.cpp:3419-3435: um, mm, cm or lambda
+            const ConfVar &v = c.get("tr_start");
+            double t = v.value;
+            if (v.units == "um") t = t * 1e-4 / c.lambda;
+            else if (v.units == "mm") t = t / 10 / c.lambda;
+            else if (v.units == "cm") t = t / c.lambda;
+            kv("tr_start", t * 2 * kPi / 2 / kPi);
+        }
+        kv("tr_init", c.get("tr_init").units == "volume" ? 1 : 0);
+        // the track box, kept in c/omega (:3436-3483)
+        for (const char *k : {"xtr1", "ytr1", "ztr1", "xtr2", "ytr2", "ztr2"})
+            kv(k, length(c, c.get(k), k[0] == 'x'));
+        kv("pmerging", str("pmerging", "off"));
+        kv("crpc", c.get("crpc").value);
+        L << "$\n";
+        L << "catching" << std::endl << "off" << std::endl;
+        L << "dump_photons" << std::endl << "off" << std::endl;
+        L << "qed" << std::endl << (p.qed ? "on" : "off") << std::endl;
+        L << "balancing" << std::endl << "off" << std::endl;
+        L << "output_mode\n" << (run.out_grid.binary ? 1 : 0) << '\n';
+        L << "#------------------------------\n";
+        const std::string polarization = str("polarization", "");
+        L << "polarization = " << polarization << "\n";
+        const double am = a0y * (a0y > a0z) + a0z * (a0z >= a0y);
+        L << "P = " << am * am / 8 * ysigma * zsigma * 8.75e9 / 1e12 << " TW\n";
+        L << "I = " << kPi * am * am * 8.75e9 / (c.lambda * c.lambda) << " W/cm^2\n";
+        L << "W = " << (a0y * a0y + a0z * a0z) * xsigma * ysigma * zsigma * kPi * std::sqrt(kPi / 2) / 4 * c.lambda *
+                           3.691e4 / 1e7
+          << " J\n";
+        L << "lambda = " << c.lambda * 1e4 << " um\n";
+        L << "xsigma = " << xsigma / 2 / kPi * c.lambda * 1e4 << " um = c*" << xsigma / 2 / kPi * c.lambda / kLightCm * 1e15
+          << " fs\n";
+        L << "ysigma = " << ysigma / 2 / kPi * c.lambda * 1e4 << " um\n";
+        L << "zsigma = " << zsigma / 2 / kPi * c.lambda * 1e4 << " um\n";
+        L << "last_t_end = " << run.out_time.back() / 2 / kPi * c.lambda * 10 << " mm\n";
+        L << "n_cr = " << kPi / (2.818e-13 * c.lambda * c.lambda) << " cm^{-3}\n";
+        L << "ne/n_cr = " << ne / (kPi / (2.818e-13 * c.lambda * c.lambda)) << "\n";
+        L << "#------------------------------\n";
+        run.log = L.str();
     }
     return true;
 }

@@ -29,6 +29,9 @@ struct Film {
     double y0, y1, z0, z1;
     double T, vx;
     int npic[3];
+    // the values the reference's film object keeps (printed in the log)
+    double log_width = 0, log_y1 = 0, log_z1 = 0, log_ne = 0, log_mcr = 1;
+    int log_npic[3] = {0, 0, 0};
 };
 
 struct Run {
@@ -44,6 +47,10 @@ struct Run {
     std::vector<double> out_time;
     OutputSet out_set;
     OutputGrid out_grid{};
+    SpectraConf spectra;
+    std::vector<double> t_end_list, period_list;   // c/omega, in config order
+    double x0fout = 0;                             // c/omega
+    std::string log;                               // the text of the log file (src/main.cpp:3611-3787)
     // moving window (src/main.cpp:2665-2679, 2932-2958, 3496-3525)
     int mwindow = 0;                 // 0 off, 1 on (mwspeed), 2 auto (mwtolerance)
     double mwspeed = 0, mwtolerance = 0;

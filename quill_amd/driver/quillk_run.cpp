@@ -10,7 +10,8 @@
 //
 // Output iterations write the reference's plane files (ex .. bz, w, inv,
 // rho, rho_p, rho_ph, jx .. jz as the config asks) unless --no-output.
-// --dry-run prints the derived run as JSON and touches no GPU;
+// --dry-run prints the derived run as JSON and touches no GPU; --write-log
+// DIR writes only the log file (src/main.cpp:3611-3787) and touches no GPU;
 // --print-conf prints the parsed variables as JSON; --dump-setup DIR writes
 // the initial particles of every species (raw little-endian doubles, rows
 // of x y z ux uy uz q, file species<slot>.f64) and the laser calls
@@ -197,13 +198,53 @@ int write_planes(const char *in, const char *dir)
     return 0;
 }
 
+// --write-particles IN DIR: the spectrum and phase-space writer on given
+// particles (a check of its file formats).  IN holds int32 n_ion, the
+// SpectraConf numbers (4 doubles deps.., 8 int32 neps.. enthp.., 2 int32
+// write_p write_ph), doubles dx dy dz lambda_cm, n_ion doubles icmr, then per
+// species (3 + n_ion) an int64 count and count x 10 doubles (x y z ux uy uz
+// g q cmr chi).
+int write_particles(const char *in, const char *dir)
+{
+    std::ifstream f(in, std::ios::binary);
+    int32_t n_ion = 0;
+    if (!f.read((char *)&n_ion, 4) || n_ion < 0 || n_ion > QK_MAX_ION) return 1;
+    SpectraConf c;
+    double d4[4];
+    int32_t i10[10];
+    if (!f.read((char *)d4, sizeof d4) || !f.read((char *)i10, sizeof i10)) return 1;
+    c.deps = d4[0]; c.deps_p = d4[1]; c.deps_ph = d4[2]; c.deps_i = d4[3];
+    c.neps = i10[0]; c.neps_p = i10[1]; c.neps_ph = i10[2]; c.neps_i = i10[3];
+    c.enthp = i10[4]; c.enthp_p = i10[5]; c.enthp_ph = i10[6]; c.enthp_i = i10[7];
+    c.write_p = i10[8] != 0;
+    c.write_ph = i10[9] != 0;
+    double g4[4];
+    if (!f.read((char *)g4, sizeof g4)) return 1;
+    std::vector<double> icmr(n_ion);
+    if (n_ion && !f.read((char *)icmr.data(), (std::streamsize)(n_ion * sizeof(double)))) return 1;
+    std::vector<std::vector<double>> recs(3 + n_ion);
+    std::vector<SpeciesRecords> sp(3 + n_ion);
+    for (int s = 0; s < 3 + n_ion; ++s) {
+        int64_t n = 0;
+        if (!f.read((char *)&n, 8) || n < 0) return 1;
+        recs[s].resize((size_t)n * 10 + 1);
+        if (n && !f.read((char *)recs[s].data(), (std::streamsize)(n * 10 * sizeof(double)))) return 1;
+        sp[s] = SpeciesRecords{recs[s].data(), n};
+    }
+    mkdir(dir, 0755);
+    PhasespaceCounters k;
+    write_spectrum_phasespace(dir, "", c, sp.data(), n_ion, icmr.data(), g4[0], g4[1], g4[2], g4[3], k);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     if (argc < 2) return usage();
     if (argc == 4 && std::strcmp(argv[1], "--write-planes") == 0) return write_planes(argv[2], argv[3]);
-    std::string conf = argv[1], out_dir, setup_dir;
+    if (argc == 4 && std::strcmp(argv[1], "--write-particles") == 0) return write_particles(argv[2], argv[3]);
+    std::string conf = argv[1], out_dir, setup_dir, log_dir;
     int steps_override = -1, device = 0, sort_every = -1, fills_nmw = 0;
     double sort_fallback = -1;
     bool fp64 = false, dry = false, global_dep = false, show_conf = false, write_outputs = true;
@@ -222,6 +263,7 @@ int main(int argc, char **argv)
         else if (s == "--device") { const char *v = next(); if (!v) return usage(); device = std::atoi(v); }
         else if (s == "--out") { const char *v = next(); if (!v) return usage(); out_dir = v; }
         else if (s == "--mw-fills") { const char *v = next(); if (!v) return usage(); fills_nmw = std::atoi(v); }
+        else if (s == "--write-log") { const char *v = next(); if (!v) return usage(); log_dir = v; }
         else return usage();
     }
 
@@ -256,6 +298,12 @@ int main(int argc, char **argv)
         print_json(run, sp, steps);
         return 0;
     }
+    if (!log_dir.empty()) {   // the log file alone, no GPU
+        mkdir(log_dir.c_str(), 0755);
+        std::ofstream flog(log_dir + "/log");
+        flog << run.log;
+        return flog ? 0 : 1;
+    }
     if (fills_nmw > 0) {
         print_fills(run, fills_nmw);
         return 0;
@@ -277,6 +325,10 @@ int main(int argc, char **argv)
     }
     if (!rc) rc = check(qk_init_boundaries(ctx), "qk_init_boundaries");
 
+    if (write_outputs) {
+        std::ofstream flog(out_dir + "/log");
+        flog << run.log;
+    }
     std::ofstream fN(out_dir + "/N"), fE(out_dir + "/energy"), fMW;
     if (run.mwindow) fMW.open(out_dir + "/mwcoordinate");
     const int batch = 100;
@@ -286,6 +338,7 @@ int main(int argc, char **argv)
     size_t next_out = 0;   // output iterations are written after their step (src/main.cpp:2344-2430)
     const size_t nf = 3 * (size_t)p.nx * p.ny * p.nz;
     int nmw = 1;           // src/main.cpp:122
+    PhasespaceCounters ps_counters;
     // check_moving_window (src/main.cpp:1953-1978) after step l; nm = 4
     auto window_moves = [&](int l, bool &moves) -> int {
         moves = false;
@@ -318,6 +371,9 @@ int main(int argc, char **argv)
         seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (int l = 0; !rc && l < nb; ++l) {
             const qk_diag &d = diag[l];
+            if (d.n_nonfinite > 0)
+                std::fprintf(stderr, "quillk-run: step %d dropped %lld particles whose momentum was not finite\n",
+                             done + l, (long long)d.n_nonfinite);
             fN << d.N_e << '\t' << d.N_p << '\t' << d.N_ph << std::endl;
             fE << d.energy_f << '\t' << d.energy_e << '\t' << d.energy_p << '\t' << d.energy_ph;
             for (int n = 0; n < p.n_ion; ++n) fE << '\t' << d.energy_ions[n];
@@ -345,6 +401,20 @@ int main(int argc, char **argv)
             if (!rc)
                 write_output_set(out_dir, iteration_suffix(run.out_time[next_out]), run.out_grid, run.out_set,
                                  E.data(), B.data(), Be.data(), rho.data(), J.data(), p.dx / p.dt);
+            // spectra and phase space of every species (src/main.cpp:2355)
+            std::vector<std::vector<double>> recs(3 + p.n_ion);
+            std::vector<SpeciesRecords> sp_recs(3 + p.n_ion);
+            for (int s = 0; !rc && s < 3 + p.n_ion; ++s) {
+                int64_t n = 0;
+                rc = check(qk_particle_count(ctx, s, &n), "qk_particle_count");
+                recs[s].resize((size_t)std::max<int64_t>(n, 1) * 10);
+                if (!rc) rc = check(qk_get_particles(ctx, s, n, recs[s].data(), nullptr, &n), "qk_get_particles");
+                sp_recs[s] = SpeciesRecords{recs[s].data(), n};
+            }
+            if (!rc)
+                write_spectrum_phasespace(out_dir, iteration_suffix(run.out_time[next_out]), run.spectra,
+                                          sp_recs.data(), p.n_ion, p.ion_cmr, p.dx, p.dy, p.dz, run.lambda_cm,
+                                          ps_counters);
             ++next_out;
         }
         // add_moving_window_particles (src/main.cpp:1750-1835, 2396-2398)

@@ -26,6 +26,8 @@ Cases
   qed_stats  QED on (photon emission, Breit-Wheeler): per-step diagnostics and
              the photon-gamma histogram of 16 runs with distinct seeds of the
              reference's own random stream (statistical parity)
+  qed_pairs  a harder QED case (about 1400 pairs per run): yields and the
+             photon and positron spectra of 16 reference runs
 """
 import os
 import subprocess
@@ -193,6 +195,26 @@ def case_qed_stats():
     print(f"qed_stats: {qed_case.SEEDS} seeds, final photons {s[:, 0].mean():.0f}, positrons {s[:, 1].mean():.1f}")
 
 
+def case_qed_pairs():
+    """The pair-rich QED case (qed_case.inputs_pairs) over qed_case.SEEDS
+    seeds of the reference's own random stream: yields and the photon and
+    positron spectra."""
+    c = qed_case.inputs_pairs()
+    summ = []
+    for seed in range(1, qed_case.SEEDS + 1):
+        h = gio.header(mode=gio.MODE_RUN, nx=c["nx"], ny=c["ny"], nz=c["nz"], solver=oracle.FDTD,
+                       pusher=oracle.VAY, qed=1, nsteps=c["nsteps"], n_particles=len(c["rec"]), seed=seed, n_ion=0,
+                       dx=c["dx"], dy=c["dx"], dz=c["dx"], dt=c["dt"], e_s=qed_case.E_S,
+                       **norms(c["dx"], c["dx"], c["dx"]))
+        _, prec, _, diag = run_ref(h, c["E"], c["B"], c["rec"], c["ids"])
+        ph, po = prec[prec[:, 8] == 0], prec[prec[:, 8] == 1]
+        summ.append(qed_case.summary_pairs(diag[-1][5], diag[-1][4], diag[-1][9], diag[-1][8], ph[:, 6], po[:, 6]))
+    np.savez_compressed(os.path.join(HERE, "qed_pairs.npz"), header=h.tobytes(), summary=np.array(summ))
+    s = np.array(summ)
+    print(f"qed_pairs: {qed_case.SEEDS} seeds, photons {s[:, 0].mean():.0f}, positrons {s[:, 1].mean():.1f}"
+          f" +- {s[:, 1].std():.1f}")
+
+
 def case_merge_stats():
     """pmerging of the reference (src/mc.cpp:202-462), "ti" and "nl", over
     merge_case.SEEDS seeds of its own random stream: run summaries."""
@@ -344,6 +366,7 @@ def main(names):
         "rates": case_rates,
         "push": case_push,
         "qed_stats": case_qed_stats,
+        "qed_pairs": case_qed_pairs,
         "fout": case_fout,
         "merge_stats": case_merge_stats,
         "mw_on": lambda: case_mw("on"),

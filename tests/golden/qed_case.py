@@ -5,6 +5,10 @@ chi ~ 1, so photons are emitted within a step and the hardest photons make
 Breit-Wheeler pairs.  The reference runs it with its own random stream for
 16 seeds (qed_stats.npz); the oracle and the GPU run it with their
 counter-based streams, and the summaries are compared as distributions.
+
+The pair case (inputs_pairs, qed_pairs.npz): u_x = 300 in B_z = 3000 for 12
+steps, chi ~ 2, about 1400 pairs per run, so the positron yield and
+spectrum are measured to a few per cent per run.
 """
 import numpy as np
 
@@ -29,6 +33,39 @@ def inputs(b0=2000.0, n=4000):
     rec[:, 7] = -1.0
     ids = np.arange(n, dtype=np.int32)
     return dict(nx=nx, ny=ny, nz=nz, dx=dx, dt=dt, E=E, B=B, rec=rec, ids=ids, nsteps=10)
+
+
+def inputs_pairs():
+    c = inputs(b0=3000.0, n=4000)
+    c["rec"][:, 3] = 300.0
+    c["nsteps"] = 12
+    return c
+
+
+PAIR_BINS = np.geomspace(1.0, 300.0, 8)   # positron gamma histogram edges
+
+
+def summary_pairs(n_ph, n_p, energy_ph, energy_p, photon_g, positron_g):
+    """Yields (photon and positron counts and energies), then the photon and
+    positron gamma histograms."""
+    return np.r_[float(n_ph), float(n_p), float(energy_ph), float(energy_p),
+                 np.histogram(photon_g, bins=BINS)[0].astype(float),
+                 np.histogram(positron_g, bins=PAIR_BINS)[0].astype(float)]
+
+
+NAMES_PAIRS = (["n_ph", "n_p", "energy_ph", "energy_p"] +
+               [f"photon hist[{a:.3g},{b:.3g})" for a, b in zip(BINS[:-1], BINS[1:])] +
+               [f"positron hist[{a:.3g},{b:.3g})" for a, b in zip(PAIR_BINS[:-1], PAIR_BINS[1:])])
+YIELDS = 4   # leading columns of summary_pairs that are yields
+
+
+def spectrum_chi2_p(z):
+    """p-value of sum z^2 over the spectrum bins (chi-square with one degree
+    of freedom per bin): the spectra agree within 3 sigma as a whole when
+    p > 0.0027."""
+    from scipy.stats import chi2
+    z = np.asarray(z, float)
+    return float(chi2.sf((z * z).sum(), len(z)))
 
 
 def summary(n_ph, n_p, energy_ph, photon_g):

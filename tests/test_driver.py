@@ -205,3 +205,56 @@ def test_unsupported_options_are_refused(tmp_path, extra, msg):
 def test_qed_defaults_on(tmp_path):
     j = json.loads(run(tmp_path, BASE.replace("qed = off\n", ""), "--dry-run").stdout)
     assert j["qed"] == 1   # find_boolean(first, "qed", true), src/main.cpp:3542
+
+
+LOG_KEYS = ["dx", "dy", "dz", "dt", "nx", "ny", "nz", "lambda", "ne", "t_end", "output_period", "t_add_mw",
+            "xsigma", "ysigma", "zsigma", "x0fout", "a0y", "a0z", "external_bz", "mwindow",
+            "e_components_for_output", "b_components_for_output", "f_envelope", "b_sign", "x0", "y0", "z0",
+            "xtarget", "ytarget", "ztarget", "phase", "lp_reflection", "f_reflection", "phi", "phi_rotate",
+            "shenergy", "shphase", "beam", "beam_particles", "Nb", "epsb", "xb", "rb", "x0b", "y0b", "phib",
+            "mwspeed", "nelflow", "vlflow", "mcrlflow", "Tlflow", "nerflow", "vrflow", "mcrrflow", "Trflow", "ions",
+            "film", "x0film", "filmwidth", "gradwidth", "gradwidth_y", "y0film", "y1film", "z0film", "z1film",
+            "nfilm", "mcr", "Tfilm", "vxfilm", "xnpic_film", "ynpic_film", "znpic_film", "n_ion_populations",
+            "icmr", "xnpic", "ynpic", "znpic", "particles_for_output", "deps", "neps", "enthp", "deps_p", "neps_p",
+            "enthp_p", "deps_ph", "neps_ph", "enthp_ph", "deps_i", "neps_i", "enthp_i", "n_sr", "n_tracks",
+            "particles_to_track", "tr_start", "tr_init", "xtr1", "ytr1", "ztr1", "xtr2", "ytr2", "ztr2", "pmerging",
+            "crpc"]
+
+
+def test_log_file(tmp_path):
+    """The log main() writes after init() (src/main.cpp:3611-3787): key and
+    value lines in the reference's order up to "$" (the order of its
+    writer: one ion population's icmr, one film), then the run switches;
+    values in lambda units and cm^-3, as python/resread.py:52-133 reads
+    them back."""
+    text = ("dx = 0.05\ndy = 0.04\ndz = 0.05\ndt = 0.025\nlambda = 0.8 um\nne = 0.02 ncr\n"
+            "xlength = 3.2\nylength = 1.6\nzlength = 1.2\nt_end = 5\noutput_period = 2.5\n"
+            "a0 = 1.5\nxsigma = 2\nysigma = 3 um\nzsigma = 2\nx0 = -1\nmwindow = off\n"
+            "film = on\nx0film = 0.2\nfilmwidth = 2.5\nnfilm = 0.01 ncr\nions = on\nmcr = 2\n"
+            "xnpic = 2\nynpic = 1\nznpic = 1\ndeps = 1 mc^2\nneps = 100\nenthp = 10\n"
+            "particles_for_output = epph\noutput_mode = binary\nsolver = fdtd\npusher = boris\n")
+    run(tmp_path, text, "--write-log", str(tmp_path / "out"))
+    lines = (tmp_path / "out" / "log").read_text().split("\n")
+    end = lines.index("$")
+    pairs = list(zip(lines[:end:2], lines[1:end:2]))
+    assert [k for k, _ in pairs] == LOG_KEYS
+    v = dict(pairs)
+    lam_cm = f32(0.8) * 1e-4
+    assert float(v["dx"]) == pytest.approx(f32(0.05), rel=1e-5)
+    assert float(v["dy"]) == pytest.approx(f32(0.04), rel=1e-5)
+    assert float(v["dt"]) == pytest.approx(f32(0.025), rel=1e-5)
+    assert (int(v["nx"]), int(v["ny"]), int(v["nz"])) == (64, 40, 24)
+    assert float(v["lambda"]) == pytest.approx(lam_cm, rel=1e-5)
+    assert float(v["ne"]) == pytest.approx(f32(0.02) * 1.11485e13 / lam_cm ** 2, rel=1e-5)
+    assert float(v["t_end"]) == pytest.approx(5, rel=1e-5) and float(v["output_period"]) == pytest.approx(2.5)
+    assert float(v["ysigma"]) == pytest.approx(f32(3) * 1e-4 / lam_cm, rel=1e-5)   # um -> lambda
+    assert float(v["a0y"]) == pytest.approx(1.5) and float(v["a0z"]) == 0
+    assert float(v["deps"]) == pytest.approx(0.511, rel=1e-5)                       # mc^2 -> MeV
+    assert v["particles_for_output"] == "epph" and v["n_ion_populations"] == "1"
+    assert float(v["icmr"]) == pytest.approx(1 / (1836.1526721 * 2), rel=1e-5)
+    assert float(v["nfilm"]) == pytest.approx(f32(0.01) * 1.11485e13 / lam_cm ** 2, rel=1e-5)
+    assert v["mwindow"] == "off" and v["n_sr"] == "1"
+    tail = lines[end + 1:]
+    for k, want in (("catching", "off"), ("dump_photons", "off"), ("output_mode", "1")):
+        assert tail[tail.index(k) + 1] == want
+    assert tail[tail.index("qed") + 1] in ("on", "off")

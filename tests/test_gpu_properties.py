@@ -241,3 +241,31 @@ def test_gauss_law_full_size_fp32():
     assert diag[-1]["n_e"] == n_live
     assert 0 < n0 - n_live < 0.01 * n0
     assert frac < 0.01
+
+
+@pytest.mark.parametrize("precision,deposit,qed", [("fp32", "tiled", False), ("fp64", "global", False),
+                                                   ("fp32", "global", True)])
+def test_nonfinite_particles_are_reported(precision, deposit, qed):
+    """Particles whose momentum is NaN or infinite (corrupt input, forced
+    from the host) are dropped by the push and counted in the step's
+    n_nonfinite, not lost silently; the others are pushed as usual."""
+    nx, ny, nz = 20, 16, 16
+    rng = np.random.default_rng(4)
+    n = 400
+    x = rng.uniform(3, nx - 4, n)
+    y = rng.uniform(3, ny - 4, n)
+    z = rng.uniform(3, nz - 4, n)
+    u = rng.normal(0, 0.1, (n, 3))
+    u[7, 0] = np.nan
+    u[11, 1] = np.inf
+    u[13, 2] = -np.inf
+    print(f"non-finite momenta {precision} {deposit} qed={qed}", flush=True)
+    kw = dict(qed=True, e_s=1e5) if qed else {}
+    with qa.Region(nx, ny, nz, DX, DX, DX, DT, precision=precision, deposit=deposit, **kw) as r:
+        r.add_particles(qa.ELECTRON, x, y, z, u[:, 0], u[:, 1], u[:, 2], np.full(n, -0.01))
+        r.init_boundaries()
+        d = r.step(2, diag=True)
+        assert d[0]["n_nonfinite"] == 3 and d[1]["n_nonfinite"] == 0
+        assert d[0]["n_e"] == n - 3 and r.count(qa.ELECTRON) == n - 3
+        E, B, J, _ = r.get_fields()
+        assert np.isfinite(E).all() and np.isfinite(B).all() and np.isfinite(J).all()

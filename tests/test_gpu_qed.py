@@ -2,9 +2,12 @@
 
 Event sequences cannot match the reference (its get_rand is a sequential
 lagged-Fibonacci stream, src/mc.cpp:12-23; the GPU uses counter-based Philox
-streams), so parity is statistical: 16 seeded GPU runs against the 16
-reference runs of tests/golden/qed_stats.npz (made by the reference's own
-code, tests/golden/make_golden.py), |z| < 4 on every summary.
+streams), so parity is statistical: 16 seeded GPU runs against 16 runs of
+the reference's own code (tests/golden/make_golden.py).  Two cases: the
+chi ~ 1 beam of qed_stats.npz (|z| < 4 on every summary) and the pair-rich
+chi ~ 2 case of qed_pairs.npz (about 1400 pairs per run): yields within 3
+sigma (|z| < 3) and the photon and positron spectra each within 3 sigma as
+a whole (sum of z^2 over the bins, chi-square p > 0.0027).
 """
 import os
 import sys
@@ -48,6 +51,38 @@ def test_qed_statistics_match_reference(precision):
     z = qed_case.zscores(mine, ref)
     bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES, z) if not abs(v) < 4]
     assert not bad, f"GPU QED statistics differ from the reference: {bad}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_qed_pairs_match_reference(precision):
+    print(f"QED pair case, {precision}, {qed_case.SEEDS} seeds", flush=True)
+    c = qed_case.inputs_pairs()
+    fx = gio.load("qed_pairs")
+    h = np.frombuffer(fx["header"].tobytes(), gio.HDR)[0]
+    mine = []
+    for seed in range(1, qed_case.SEEDS + 1):
+        with qa.Region(c["nx"], c["ny"], c["nz"], c["dx"], c["dx"], c["dx"], c["dt"], solver="fdtd", pusher="vay",
+                       precision=precision, qed=True, e_s=qed_case.E_S, seed=seed, norm_n=float(h["norm_n"]),
+                       norm_ef=float(h["norm_ef"]), norm_ep=float(h["norm_ep"])) as r:
+            r.set_fields(c["E"], c["B"])
+            r.add_records(c["rec"], c["ids"])
+            r.init_boundaries()
+            d = r.step(c["nsteps"], diag=True)[-1]
+            ph, _ = r.particles(qa.PHOTON)
+            po, _ = r.particles(qa.POSITRON)
+            mine.append(qed_case.summary_pairs(d["n_ph"], d["n_p"], d["energy_ph"], d["energy_p"], ph[:, 6],
+                                               po[:, 6]))
+    z = qed_case.zscores(np.array(mine), fx["summary"])
+    y = qed_case.YIELDS
+    print("yields z: " + ", ".join(f"{n} {v:.2f}" for n, v in zip(qed_case.NAMES_PAIRS[:y], z[:y])), flush=True)
+    bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES_PAIRS[:y], z[:y]) if not abs(v) < 3]
+    assert not bad, f"GPU QED yields differ from the reference: {bad}"
+    nb = len(qed_case.BINS) - 1
+    for name, zz in (("photon", z[y:y + nb]), ("positron", z[y + nb:])):
+        p = qed_case.spectrum_chi2_p(zz)
+        print(f"{name} spectrum: chi-square p = {p:.3g}", flush=True)
+        assert p > 0.0027, (name, p, np.round(zz, 2))
 
 
 @pytest.mark.gpu

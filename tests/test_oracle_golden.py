@@ -184,6 +184,40 @@ def test_oracle_qed_statistics_match_reference():
     assert not bad, f"oracle QED statistics differ from the reference: {bad}"
 
 
+def test_oracle_qed_pairs_match_reference():
+    """The pair-rich case (about 1400 Breit-Wheeler pairs per run) of the
+    oracle against 16 runs of the reference: photon and positron yields and
+    energies within 3 sigma (|z| < 3 each), and the photon and positron
+    gamma spectra each within 3 sigma as a whole (sum of z^2 over the bins,
+    chi-square p > 0.0027)."""
+    import qed_case
+    c = qed_case.inputs_pairs()
+    ref = gio.load("qed_pairs")["summary"]
+    h = hdr(gio.load("qed_pairs"))
+    mine = []
+    for seed in range(1, qed_case.SEEDS + 1):
+        o = oracle.Oracle(c["nx"], c["ny"], c["nz"], c["dx"], c["dx"], c["dx"], c["dt"], solver=oracle.FDTD,
+                          pusher=oracle.VAY, qed=True, e_s=qed_case.E_S, seed=seed, norm_n=float(h["norm_n"]),
+                          norm_ef=float(h["norm_ef"]), norm_ep=float(h["norm_ep"]))
+        o.field(oracle.E)[:] = c["E"]
+        o.field(oracle.B)[:] = c["B"]
+        o.add_particles(c["rec"], c["ids"])
+        o.init_boundaries()
+        d = o.step(c["nsteps"], diag=True)[-1]
+        rec, _ = o.particles()
+        mine.append(qed_case.summary_pairs(d.n_ph, d.n_p, d.energy_ph, d.energy_p, rec[rec[:, 8] == 0][:, 6],
+                                           rec[rec[:, 8] == 1][:, 6]))
+    z = qed_case.zscores(np.array(mine), ref)
+    assert ref[:, 1].mean() > 1000
+    y = qed_case.YIELDS
+    bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES_PAIRS[:y], z[:y]) if not abs(v) < 3]
+    assert not bad, f"oracle QED yields differ from the reference: {bad}"
+    nb = len(qed_case.BINS) - 1
+    for name, zz in (("photon", z[y:y + nb]), ("positron", z[y + nb:])):
+        p = qed_case.spectrum_chi2_p(zz)
+        assert p > 0.0027, (name, p, np.round(zz, 2))
+
+
 @pytest.mark.parametrize("mode", ["ti", "nl"])
 def test_oracle_merging_matches_reference(mode):
     """pmerging (src/mc.cpp:202-462) of the oracle against 16 merges of the
