@@ -350,27 +350,31 @@ def adjoint_check(steps, every):
                              "checkpoint_every": small.every, "checks": checks}}
 
 
-def qed_cascade(steps, seed=1, precision="fp32"):
-    """The QED cascade case on one GPU: 128^3 cells spanning a 4 lambda cube,
-    five-point solver, Vay pusher, photon emission, Breit-Wheeler pairs and
-    vacuum pairs on; lambda = 1 um (e_s = lambda / lambda_C).  Two
-    counter-propagating circularly polarised pulses (peak |a| = 1000, waist
-    1.5 lambda, duration 5 lambda/c) are set as f_init_cos initial conditions
-    centred on the box (longer than the box, so they start overlapped and the
-    box holds their central 4 lambda); 1e4 electrons at rest, uniform in the
-    central lambda^3.  dt = dx / 4: the five-point solver's interior update
-    is unstable above about dt = 0.41 dx in 3-D (its grid-scale mode, k = pi
-    on every axis, has the difference factor 2 (b - 2a) with a < 0), as the
-    reference's is.  The field fills the box from the first
-    step, so the cascade grows faster than from pulses entering through the
-    walls; the reference's "ti" merging (pmerging, src/mc.cpp:202-462) holds
-    each species at crnp = 1e7 macro-particles.  Throughput counts every
-    live particle (electrons, positrons, photons) pushed per step."""
+def qed_cascade_run(steps, seed, precision="fp32", plane=4):
+    """One run of the QED cascade case on one GPU: 128^3 cells spanning a
+    4 lambda cube, five-point solver, Vay pusher, photon emission,
+    Breit-Wheeler pairs and vacuum pairs on; lambda = 1 um
+    (e_s = lambda / lambda_C, src/main.cpp:2276).  dt = dx / 4: the
+    five-point solver's interior update is unstable above about dt = 0.41 dx
+    in 3-D (its grid-scale mode, k = pi on every axis, has the difference
+    factor 2 (b - 2a) with a < 0), as the reference's is.
+
+    Two counter-propagating circularly polarised pulses (f_init_cos, peak
+    |a| = 1000, waist 1.5 lambda, duration 5 lambda/c, 8.4 lambda long) are
+    injected from -x and +x through laser sources at the planes `plane` and
+    nx - `plane`; their peaks meet at the centre at step 0: the part of each
+    pulse that has already entered (its total-field side of its source
+    plane) is set as the initial field, the rest enters through the planes
+    during the run, so the centre holds the colliding peaks for the whole
+    run.  1e4 electrons at rest, uniform in the central lambda^3; the
+    reference's "ti" merging holds each species at crnp = 1e7
+    macro-particles.  Throughput counts every live particle (electrons,
+    positrons, photons) pushed per step."""
     nx = 128
     lam = TWO_PI
     d = 4 * lam / nx
     dt = 0.25 * d
-    e_s = 1e-4 / 2.4263086e-10    # lambda / lambda_C for lambda = 1 um (src/main.cpp:2276)
+    e_s = 1e-4 / 2.4263086e-10
     a = 1000 / math.sqrt(2)
     xsigma, waist = 2.5 * lam, 1.5 * lam
     rng = np.random.default_rng(seed)
@@ -380,11 +384,19 @@ def qed_cascade(steps, seed=1, precision="fp32"):
     r = qa.Region(nx, nx, nx, d, d, d, dt, solver="five_point", pusher="vay", precision=precision, qed=True,
                   e_s=e_s, seed=seed, sort_every=0, merging="ti", crnp=crnp)
     try:
-        # propagation along -x1 (the pulse frame's axis), B of the sign that
-        # makes the Poynting vector point along it (b_sign = 1)
-        for x1 in (-1.0, 1.0):
-            r.init_laser_cos(a0y=a, a0z=a, xsigma=xsigma, ysigma=waist, zsigma=waist, x0=nx * d / 2,
-                             b_sign=1, x1=x1, append=True)
+        # pulse moving +x (x1 = -1) enters through side 0, the other through
+        # side 1; each one's initial field only on its total-field side
+        E, B = np.zeros((nx, nx, nx, 3)), np.zeros((nx, nx, nx, 3))
+        for side, x1 in ((0, -1.0), (1, 1.0)):
+            L = dict(a0y=a, a0z=a, xsigma=xsigma, ysigma=waist, zsigma=waist, x0=nx * d / 2, b_sign=1, x1=x1)
+            r.init_laser_cos(append=False, **L)
+            El, Bl, _, _ = r.get_fields()
+            sl = slice(plane, None) if side == 0 else slice(None, nx - plane)
+            E[sl] += El[sl]
+            B[sl] += Bl[sl]
+            r.add_laser_source(side, plane if side == 0 else nx - plane, **L)
+        r.set_fields(E, B)
+        del E, B, El, Bl
         z = np.zeros(n0)
         r.add_particles(qa.ELECTRON, pos[:, 0], pos[:, 1], pos[:, 2], z, z, z, np.full(n0, -1e-3))
         r.init_boundaries()
@@ -396,16 +408,42 @@ def qed_cascade(steps, seed=1, precision="fp32"):
     finally:
         r.close()
     pushed = sum(x["n_e"] + x["n_p"] + x["n_ph"] for x in diag[:-1]) + n0
-    last = diag[-1]
+    return sec, pushed, diag
+
+
+def qed_cascade(steps, seeds, precision="fp32"):
+    """The QED cascade case over `seeds` seeds (1..seeds): throughput of each
+    run, and the growth of the electron, positron and photon counts as mean
+    and standard deviation over the seeds."""
+    runs = []
+    for seed in range(1, seeds + 1):
+        sec, pushed, diag = qed_cascade_run(steps, seed, precision)
+        runs.append((sec, pushed, diag))
+        last = diag[-1]
+        log(f"cascade seed {seed}: {sec:.2f} s, electrons {last['n_e']}, positrons {last['n_p']}, "
+            f"photons {last['n_ph']}")
+    sec = sum(x[0] for x in runs)
+    pushed = sum(x[1] for x in runs)
+    marks = sorted(set(list(range(0, steps, max(1, steps // 8))) + [steps - 1]))
+    growth = []
+    for k in marks:
+        v = np.array([[x[2][k]["n_e"], x[2][k]["n_p"], x[2][k]["n_ph"]] for x in runs], float)
+        growth.append({"step": k + 1, "mean": v.mean(0).tolist(), "std": v.std(0, ddof=1).tolist() if seeds > 1 else None})
+    fin = np.array([[x[2][-1]["n_e"], x[2][-1]["n_p"], x[2][-1]["n_ph"], x[2][-1]["energy_ph"]] for x in runs], float)
     return {"metric": "QED cascade particle-steps/s on 1 MI355X", "value": pushed / sec,
             "unit": "particle-steps/s", "higher_is_better": True, "n_gpus": 1, "steps": steps,
-            "ms_per_step": sec * 1e3 / steps, "dtype": "f32" if precision == "fp32" else "f64", "data": "synthetic",
-            "config": {"workload": "qed-cascade-five-vay-128", "grid": [nx, nx, nx], "dx_lambda": 4 / nx,
-                       "dt_lambda": 1 / nx, "a0": 1000, "waist_lambda": 1.5, "duration_lambda": 5.0,
-                       "seed_electrons": n0, "rng_seed": seed, "merging": "ti", "crnp": crnp},
-            "final": {"electrons": last["n_e"], "positrons": last["n_p"], "photons": last["n_ph"],
-                      "photon_energy": last["energy_ph"]},
-            "growth": [[k, diag[k]["n_e"], diag[k]["n_p"], diag[k]["n_ph"]] for k in range(0, steps, max(1, steps // 8))]}
+            "ms_per_step": sec * 1e3 / (steps * seeds), "dtype": "f32" if precision == "fp32" else "f64",
+            "data": "synthetic",
+            "config": {"workload": "qed-cascade-five-vay-128", "grid": [128, 128, 128], "dx_lambda": 4 / 128,
+                       "dt_lambda": 1 / 128, "a0": 1000, "polarisation": "circular", "waist_lambda": 1.5,
+                       "duration_lambda": 5.0, "injection": "laser sources at x planes 4 and 124, peaks meeting "
+                       "at the centre at step 0", "seed_electrons": 10000, "rng_seeds": list(range(1, seeds + 1)),
+                       "merging": "ti", "crnp": 1e7},
+            "final_mean": {"electrons": fin[:, 0].mean(), "positrons": fin[:, 1].mean(), "photons": fin[:, 2].mean(),
+                           "photon_energy": fin[:, 3].mean()},
+            "final_std": {"electrons": fin[:, 0].std(ddof=1), "positrons": fin[:, 1].std(ddof=1),
+                          "photons": fin[:, 2].std(ddof=1)} if seeds > 1 else None,
+            "growth": growth}
 
 
 def push_roofline(run, steps, ncell, workload):
@@ -496,9 +534,10 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=300)
     ap.add_argument("--qed-cascade", action="store_true",
                     help="only run the QED cascade case (128^3, two colliding CP pulses, a0 = 1000)")
+    ap.add_argument("--seeds", type=int, default=16, help="seeds of the QED cascade case")
     args = ap.parse_args()
     if args.qed_cascade:
-        print(json.dumps(qed_cascade(args.steps if args.steps != 20 else 400)), flush=True)
+        print(json.dumps(qed_cascade(args.steps if args.steps != 20 else 400, args.seeds)), flush=True)
         return
     if args.adjoint:
         print(json.dumps(adjoint_check(args.adjoint_steps, 25)), flush=True)

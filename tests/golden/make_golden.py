@@ -215,6 +215,34 @@ def case_qed_pairs():
           f" +- {s[:, 1].std():.1f}")
 
 
+def case_qed_cascade():
+    """The reduced colliding-pulse cascade (qed_case.inputs_cascade) over
+    qed_case.SEEDS seeds of the reference's own random stream: the two
+    pulses from the reference's f_init_cos (appended), then its step with
+    photon emission and Breit-Wheeler pairs on."""
+    c = qed_case.inputs_cascade()
+    nx, ny, nz, d = c["nx"], c["ny"], c["nz"], c["dx"]
+    E = np.zeros((nx, ny, nz, 3))
+    B = np.zeros((nx, ny, nz, 3))
+    for L in c["lasers"]:
+        hl = gio.header(mode=gio.MODE_LASER, nx=nx, ny=ny, nz=nz, dx=d, dy=d, dz=d, dt=c["dt"], laser=L)
+        El, Bl = run_ref(hl)
+        E += El
+        B += Bl
+    summ = []
+    for seed in range(1, qed_case.SEEDS + 1):
+        h = gio.header(mode=gio.MODE_RUN, nx=nx, ny=ny, nz=nz, solver=oracle.FIVE_POINT, pusher=oracle.VAY, qed=1,
+                       nsteps=c["nsteps"], n_particles=len(c["rec"]), seed=seed, n_ion=0, dx=d, dy=d, dz=d,
+                       dt=c["dt"], e_s=qed_case.E_S, **norms(d, d, d))
+        _, prec, _, diag = run_ref(h, E, B, c["rec"], c["ids"])
+        last = dict(zip(gio.DIAG_NAMES, diag[-1]))
+        ph, po = prec[prec[:, 8] == 0], prec[prec[:, 8] == 1]
+        summ.append(qed_case.summary_cascade(last, ph[:, 6], po[:, 6]))
+        print(f"qed_cascade seed {seed}: electrons {last['n_e']:.0f}, positrons {last['n_p']:.0f}, "
+              f"photons {last['n_ph']:.0f}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "qed_cascade.npz"), header=h.tobytes(), summary=np.array(summ))
+
+
 def case_merge_stats():
     """pmerging of the reference (src/mc.cpp:202-462), "ti" and "nl", over
     merge_case.SEEDS seeds of its own random stream: run summaries."""
@@ -367,6 +395,7 @@ def main(names):
         "push": case_push,
         "qed_stats": case_qed_stats,
         "qed_pairs": case_qed_pairs,
+        "qed_cascade": case_qed_cascade,
         "fout": case_fout,
         "merge_stats": case_merge_stats,
         "mw_on": lambda: case_mw("on"),

@@ -45,6 +45,56 @@ def inputs_pairs():
 PAIR_BINS = np.geomspace(1.0, 300.0, 8)   # positron gamma histogram edges
 
 
+def inputs_cascade():
+    """A reduced QED cascade of the colliding-pulse kind: 32^3 cells over a
+    2 lambda cube (lambda = 1 um), five-point solver, Vay pusher, dt = dx / 4.
+    Two counter-propagating circularly polarised pulses (f_init_cos, peak
+    |a| = 1000, waist 1.5 lambda, duration 5 lambda/c) centred on the box as
+    initial conditions, the reference's way of setting a laser; 1000
+    electrons at rest, uniform in the central (lambda / 2)^3; 200 steps.
+    About 1800 positrons and 3e4 photons per run.  Returns the set-up and the
+    two pulses' f_init_cos argument vectors."""
+    n = 32
+    lam = 2 * PI
+    d = 2 * lam / n
+    dt = 0.25 * d
+    a = 1000 / np.sqrt(2)
+    lasers = []
+    for x1 in (-1.0, 1.0):
+        L = np.zeros(16)
+        # a0y a0z xsigma ysigma zsigma x0 b_sign x1 phase y0 z0 append phi xt yt zt
+        L[:12] = [a, a, 2.5 * lam, 1.5 * lam, 1.5 * lam, n * d / 2, 1, x1, 0.0, 0.0, 0.0, 1]
+        lasers.append(L)
+    nseed = 1000
+    rng = np.random.default_rng(2025)
+    rec = np.zeros((nseed, 8))
+    rec[:, 0:3] = n / 2 + (rng.random((nseed, 3)) - 0.5) * (0.5 * lam / d)
+    rec[:, 6] = -1e-3
+    rec[:, 7] = -1.0
+    ids = np.arange(nseed, dtype=np.int32)
+    return dict(nx=n, ny=n, nz=n, dx=d, dt=dt, lasers=lasers, rec=rec, ids=ids, nsteps=200)
+
+
+CASCADE_BINS = np.geomspace(1.0, 3000.0, 9)   # photon gamma histogram edges of the cascade
+CASCADE_PAIR_BINS = np.geomspace(1.0, 3000.0, 7)   # positron gamma histogram edges of the cascade
+
+
+def summary_cascade(d, photon_g, positron_g):
+    """Yields of the cascade (electron, positron and photon counts and
+    energies from the last step's diagnostics d) and the photon and positron
+    gamma histograms."""
+    return np.r_[float(d["n_e"]), float(d["n_p"]), float(d["n_ph"]), float(d["energy_e"]), float(d["energy_p"]),
+                 float(d["energy_ph"]), np.histogram(photon_g, bins=CASCADE_BINS)[0].astype(float),
+                 np.histogram(positron_g, bins=CASCADE_PAIR_BINS)[0].astype(float)]
+
+
+NAMES_CASCADE = (["n_e", "n_p", "n_ph", "energy_e", "energy_p", "energy_ph"] +
+                 [f"photon hist[{a:.3g},{b:.3g})" for a, b in zip(CASCADE_BINS[:-1], CASCADE_BINS[1:])] +
+                 [f"positron hist[{a:.3g},{b:.3g})" for a, b in zip(CASCADE_PAIR_BINS[:-1],
+                                                                    CASCADE_PAIR_BINS[1:])])
+YIELDS_CASCADE = 6
+
+
 def summary_pairs(n_ph, n_p, energy_ph, energy_p, photon_g, positron_g):
     """Yields (photon and positron counts and energies), then the photon and
     positron gamma histograms."""

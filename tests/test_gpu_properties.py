@@ -196,9 +196,12 @@ def gauss_residual(E, rho, lo=3):
     return g[lo:-lo - 1, lo:-lo - 1, lo:-lo - 1]
 
 
-@pytest.mark.parametrize("precision,deposit,tol", [("fp64", "global", 1e-11), ("fp64", "tiled", 1e-11),
-                                                   ("fp32", "tiled", 2e-5)])
+@pytest.mark.parametrize("precision,deposit,tol", [("fp64", "global", 1e-10), ("fp64", "tiled", 1e-10),
+                                                   ("fp32", "tiled", 2e-4)])
 def test_gauss_law_invariant(precision, deposit, tol):
+    """|div E - rho - (div E - rho)_0| <= tol max|rho| after 20 steps: FP64
+    round-off (1e-10) and FP32 round-off (2e-4: E holds about 7 significant
+    digits and each step adds the current of every particle to it)."""
     shape = (40, 32, 32)
     _, inputs = run_pair(*shape, "ndfx", "vay", 0, sigma=0.6, ppc=4)
     print(f"gauss {precision} {deposit}", flush=True)
@@ -212,7 +215,7 @@ def test_gauss_law_invariant(precision, deposit, tol):
     r.close()
     scale = np.abs(rho1).max()
     assert scale > 0
-    assert np.abs(g1 - g0).max() < tol * scale * 10
+    assert np.abs(g1 - g0).max() < tol * scale
 
 
 def test_gauss_law_full_size_fp32():

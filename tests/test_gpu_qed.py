@@ -113,3 +113,43 @@ def test_vacuum_pair_rate():
     mean, sd = cells * p, np.sqrt(cells * p * (1 - p))
     z = (counts[:, 1].mean() - mean) / (sd / np.sqrt(len(counts)))
     assert mean > 20 and abs(z) < 4, (counts[:, 1], mean, z)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_qed_cascade_matches_reference(precision):
+    """The reduced colliding-pulse cascade (qed_case.inputs_cascade: 32^3
+    cells, two circularly polarised pulses of peak |a| = 1000 from the
+    device's f_init_cos, five-point + Vay, 200 steps, about 1600 pairs per
+    run) on the GPU against 16 runs of the reference: electron, positron and
+    photon yields and energies within 3 sigma (|z| < 3 each), the photon and
+    positron gamma spectra each within 3 sigma as a whole (chi-square
+    p > 0.0027)."""
+    print(f"QED cascade case, {precision}, {qed_case.SEEDS} seeds", flush=True)
+    c = qed_case.inputs_cascade()
+    fx = gio.load("qed_cascade")
+    h = np.frombuffer(fx["header"].tobytes(), gio.HDR)[0]
+    n, d = c["nx"], c["dx"]
+    mine = []
+    for seed in range(1, qed_case.SEEDS + 1):
+        with qa.Region(n, n, n, d, d, d, c["dt"], solver="five_point", pusher="vay", precision=precision, qed=True,
+                       e_s=qed_case.E_S, seed=seed, norm_n=float(h["norm_n"]), norm_ef=float(h["norm_ef"]),
+                       norm_ep=float(h["norm_ep"])) as r:
+            for L in c["lasers"]:
+                r.init_laser_cos(*L)
+            r.add_records(c["rec"], c["ids"])
+            r.init_boundaries()
+            last = r.step(c["nsteps"], diag=True)[-1]
+            ph, _ = r.particles(qa.PHOTON)
+            po, _ = r.particles(qa.POSITRON)
+            mine.append(qed_case.summary_cascade(last, ph[:, 6], po[:, 6]))
+    z = qed_case.zscores(np.array(mine), fx["summary"])
+    y = qed_case.YIELDS_CASCADE
+    print("yields z: " + ", ".join(f"{nm} {v:.2f}" for nm, v in zip(qed_case.NAMES_CASCADE[:y], z[:y])), flush=True)
+    bad = [(nm, round(float(v), 2)) for nm, v in zip(qed_case.NAMES_CASCADE[:y], z[:y]) if not abs(v) < 3]
+    assert not bad, f"GPU cascade yields differ from the reference: {bad}"
+    nb = len(qed_case.CASCADE_BINS) - 1
+    for name, zz in (("photon", z[y:y + nb]), ("positron", z[y + nb:])):
+        p = qed_case.spectrum_chi2_p(zz)
+        print(f"{name} spectrum: chi-square p = {p:.3g}", flush=True)
+        assert p > 0.0027, (name, p, np.round(zz, 2))

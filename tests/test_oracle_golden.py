@@ -240,3 +240,45 @@ def test_oracle_merging_matches_reference(mode):
     assert not bad, f"oracle merging differs from the reference: {bad}"
     np.testing.assert_allclose(mine[:, 1::3], ref[:, 1::3], rtol=1e-12)
     assert (ref[:, 0] < len(c["rec"])).all()
+
+
+def _oracle_cascade_run(seed):
+    """One seed of the reduced cascade on the oracle (a worker process)."""
+    import qed_case
+    c = qed_case.inputs_cascade()
+    h = hdr(gio.load("qed_cascade"))
+    n, d = c["nx"], c["dx"]
+    o = oracle.Oracle(n, n, n, d, d, d, c["dt"], solver=oracle.FIVE_POINT, pusher=oracle.VAY, qed=True,
+                      e_s=qed_case.E_S, seed=seed, norm_n=float(h["norm_n"]), norm_ef=float(h["norm_ef"]),
+                      norm_ep=float(h["norm_ep"]))
+    for L in c["lasers"]:
+        o.laser_cos(*L)
+    o.add_particles(c["rec"], c["ids"])
+    o.init_boundaries()
+    last = o.step(c["nsteps"], diag=True)[-1]
+    rec, _ = o.particles()
+    dg = {k: getattr(last, k) for k in ("n_e", "n_p", "n_ph", "energy_e", "energy_p", "energy_ph")}
+    return qed_case.summary_cascade(dg, rec[rec[:, 8] == 0][:, 6], rec[rec[:, 8] == 1][:, 6])
+
+
+def test_oracle_qed_cascade_matches_reference():
+    """The reduced colliding-pulse cascade (32^3 cells, two circularly
+    polarised pulses of peak |a| = 1000, five-point + Vay, 200 steps, about
+    1600 pairs and 3e4 photons per run) of the oracle against 16 runs of the
+    reference: electron, positron and photon yields and energies within 3
+    sigma (|z| < 3 each), the photon and positron gamma spectra each within 3
+    sigma as a whole (chi-square p > 0.0027)."""
+    from concurrent.futures import ProcessPoolExecutor
+    import qed_case
+    ref = gio.load("qed_cascade")["summary"]
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        mine = np.array(list(ex.map(_oracle_cascade_run, range(1, qed_case.SEEDS + 1))))
+    assert ref[:, 1].mean() > 1000 and ref[:, 2].mean() > 1e4
+    z = qed_case.zscores(mine, ref)
+    y = qed_case.YIELDS_CASCADE
+    bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES_CASCADE[:y], z[:y]) if not abs(v) < 3]
+    assert not bad, f"oracle cascade yields differ from the reference: {bad}"
+    nb = len(qed_case.CASCADE_BINS) - 1
+    for name, zz in (("photon", z[y:y + nb]), ("positron", z[y + nb:])):
+        p = qed_case.spectrum_chi2_p(zz)
+        assert p > 0.0027, (name, p, np.round(zz, 2))
