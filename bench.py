@@ -48,7 +48,7 @@ WORKLOADS = {
     "uniform-ndfx-vay-256x128x128": dict(grid=(256, 128, 128), solver="ndfx", pusher="vay", density=0.01, a0=1.0,
                                          waist=2.0, duration=4.0),
     "uniform-fdtd-boris-64x32x32": dict(grid=(64, 32, 32), solver="fdtd", pusher="boris", density=0.01, a0=1.0,
-                                        waist=2.0, duration=4.0),
+                                        waist=2.0, duration=4.0, source=True),
     "wakefield-fdtd-boris-512x256x256": dict(grid=(512, 256, 256), solver="fdtd", pusher="boris", density=0.001,
                                              a0=4.0, waist=6.0, duration=8.0),
 }
@@ -91,6 +91,19 @@ HEADLINE = "wakefield-fdtd-boris-512x256x256"
 CPU_SAMPLE_GRID = {"wakefield-fdtd-boris-512x256x256": (64, 256, 256)}
 
 
+SOURCE_PLANE = 2
+
+
+def source_params(w, dx):
+    """A laser source at x plane 2 (side 0) whose pulse front stands at that
+    plane when the run starts: the pulse of laser_params with its centre at
+    2 dx - xs, moving +x, entering through x-min."""
+    lp = laser_params(w, dx)
+    lp["x0"] = SOURCE_PLANE * dx - lp["xsigma"] * 2 * math.sqrt(2 * math.pi) / 3
+    lp["x1"] = -1.0
+    return lp
+
+
 def make_region(w, precision, deposit, sort_every=20, sort_fallback=SORT_FALLBACK):
     nx, ny, nz = w["grid"]
     dx = DX_LAMBDA * TWO_PI
@@ -98,7 +111,10 @@ def make_region(w, precision, deposit, sort_every=20, sort_fallback=SORT_FALLBAC
     r = qa.Region(nx, ny, nz, dx, dx, dx, dt, solver=w["solver"], pusher=w["pusher"], precision=precision,
                   deposit=deposit, sort_every=sort_every, sort_fallback=sort_fallback, device=0)
     n = r.load_uniform_plasma(qa.ELECTRON, PPC, w["density"], SIGMA_U, SEED)
-    r.init_laser_cos(**laser_params(w, dx))
+    if w.get("source"):
+        r.add_laser_source(0, SOURCE_PLANE, **source_params(w, dx))
+    else:
+        r.init_laser_cos(**laser_params(w, dx))
     r.init_boundaries()
     return r, n
 
@@ -479,8 +495,9 @@ def workload_block(name, steps, warmup, fp64_steps, err_steps, moments_steps=0):
         "value": value, "unit": "particle-steps/s", "steps": steps, "warmup": warmup, "ms_per_step": ms_step,
         "config": {"workload": name, "grid": [nx, ny, nz], "cells": ncell, "ppc": PPC, "particles": run["n_start"],
                    "solver": w["solver"], "pusher": w["pusher"], "qed": False, "density_ncr": w["density"],
-                   "laser_a0": w["a0"], "deposit": "tiled", "sort_every": se, "sort_fallback": SORT_FALLBACK,
-                   "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
+                   "laser_a0": w["a0"], "laser": "source at x plane 2 (enters through x-min)" if w.get("source")
+                   else "f_init_cos initial condition, rear edge at x-min", "deposit": "tiled", "sort_every": se,
+                   "sort_fallback": SORT_FALLBACK, "dx_lambda": DX_LAMBDA, "dt_lambda": DT_LAMBDA},
         "roofline": push_roofline(run, steps, ncell, name),
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
                      "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
@@ -580,8 +597,10 @@ def main():
         line["uniform_ndfx_vay_256x128x128"] = workload_block(
             c2, args.steps, args.warmup, 0 if args.no_compare else min(args.fp64_steps, args.steps),
             0 if args.no_compare else min(50, args.steps), 0 if args.no_compare else 500)
-        # the 64x32x32 FDTD / Boris case: the GPU, and the reference's own
-        # 200 steps on one host core
+        # the 64x32x32 FDTD / Boris case: the GPU (laser entering through a
+        # source at x-min), and the reference's own 200 steps on one host
+        # core (the reference has no source: its laser is f_init_cos with
+        # the pulse's rear edge at x-min; the per-step work is the same)
         c1 = "uniform-fdtd-boris-64x32x32"
         log(c1)
         blk = workload_block(c1, 200, 5, 0, 0)
