@@ -141,6 +141,14 @@ int qk_get_particles(qk_ctx *ctx, int species, int64_t cap, double *rec10, int32
 /* compute_rho (src/fout.cpp:7-39): node charge of electrons, positrons and
  * photons in the three slots of rho (AoS [nx][ny][nz][3]) */
 int qk_compute_rho(qk_ctx *ctx, double *rho);
+/* qk_compute_rho without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The node charges
+ * accumulate in a fixed device array of the library (48 MiB on each GPU it
+ * runs on, held until the process ends), so grids of more than 2097152 cells
+ * are refused with QK_ERR_CAPACITY.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (contexts used from several threads) race. */
+int qk_compute_rho_noalloc(qk_ctx *ctx, double *rho);
 /* A laser entering through an x plane.  No counterpart in the reference,
  * whose lasers are initial conditions only (f_init_cos, src/init.cpp:6-210,
  * called from src/main.cpp:1150-1227): here the same pulse (L as in

@@ -31,9 +31,9 @@ SPECIES_NAMES = {"electron": ELECTRON, "positron": POSITRON, "photon": PHOTON}
 
 EXPORTS = ["qk_create", "qk_destroy", "qk_last_error", "qk_set_fields", "qk_get_fields", "qk_add_particles",
            "qk_load_uniform_plasma", "qk_init_laser_cos", "qk_init_boundaries", "qk_step", "qk_sync",
-           "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_last_timing", "qk_last_fallback",
-           "qk_get_stream", "qk_pmerging", "qk_moving_window", "qk_max_w", "qk_fill_cells", "qk_tape_begin",
-           "qk_adjoint", "qk_add_laser_source", "qk_tangent_begin", "qk_tangent_get"]
+           "qk_particle_count", "qk_get_particles", "qk_compute_rho", "qk_compute_rho_noalloc", "qk_last_timing",
+           "qk_last_fallback", "qk_get_stream", "qk_pmerging", "qk_moving_window", "qk_max_w", "qk_fill_cells",
+           "qk_tape_begin", "qk_adjoint", "qk_add_laser_source", "qk_tangent_begin", "qk_tangent_get"]
 
 
 class QuillError(RuntimeError):
@@ -96,6 +96,7 @@ def load_library(path=LIB_PATH):
     L.qk_particle_count.argtypes = [vp, C.c_int, i64p]
     L.qk_get_particles.argtypes = [vp, C.c_int, C.c_int64, dp, ip, i64p]
     L.qk_compute_rho.argtypes = [vp, dp]
+    L.qk_compute_rho_noalloc.argtypes = [vp, dp]
     L.qk_add_laser_source.argtypes = [vp, dp, C.c_int, C.c_int]
     L.qk_last_timing.argtypes = [vp, dp, dp, dp]
     L.qk_last_fallback.argtypes = [vp, dp]
@@ -279,6 +280,13 @@ class Region:
     def compute_rho(self):
         rho = np.zeros(self.shape + (3,))
         _check(load_library().qk_compute_rho(self._h, _dp(rho)), "qk_compute_rho")
+        return rho
+
+    def compute_rho_noalloc(self):
+        """compute_rho through qk_compute_rho_noalloc: no device allocation
+        (the library's fixed scratch; grids of at most 2**21 cells)."""
+        rho = np.zeros(self.shape + (3,))
+        _check(load_library().qk_compute_rho_noalloc(self._h, _dp(rho)), "qk_compute_rho_noalloc")
         return rho
 
     def pmerging(self, mode, ppd):

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,38 @@ int fail(int code, const std::string &msg)
         int rc_ = (expr);         \
         if (rc_ != QK_OK) return rc_; \
     } while (0)
+
+// Device scratch of qk_compute_rho_noalloc: the three FP64 node-charge
+// arrays of a grid of up to kNoallocCells cells (48 MiB, held on each GPU the
+// library runs on until the process ends).  In host code `scratch` is a
+// host-side stand-in, not a device address: each device's copy is resolved
+// once, by qk_create with that device current, into g_scratch_ptr[device],
+// and only that pointer reaches kernels and copies.
+constexpr int64_t kNoallocCells = int64_t(1) << 21;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[3 * kNoallocCells];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// this device's copy of `scratch`; the device must be current
+int resolve_scratch(int dev)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return QK_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(QK_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)ptr;
+    return QK_OK;
+}
+
+double *scratch_ptr(int dev)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
 
 // the arrays of one particle buffer (structure of arrays)
 struct SoA {
@@ -1127,9 +1160,11 @@ int qk_create(const qk_params *params, qk_ctx **out)
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) return fail(QK_ERR_HIP, "no HIP device available");
-    if (p.device < 0 || p.device >= ndev) return fail(QK_ERR_INVALID, "device ordinal out of range");
+    if (p.device < 0 || p.device >= ndev || p.device >= kMaxDevices)
+        return fail(QK_ERR_INVALID, "device ordinal out of range");
     e = hipSetDevice(p.device);
     if (e != hipSuccess) return fail(QK_ERR_HIP, hipGetErrorString(e));
+    if (int rc = resolve_scratch(p.device)) return rc;
 
     qk_ctx *c = new qk_ctx;
     c->p = p;
@@ -1467,6 +1502,32 @@ int qk_compute_rho(qk_ctx *c, double *rho)
     const hipError_t ef = hipFree(d);
     HIP_TRY(e);
     HIP_TRY(ef);
+    return QK_OK;
+}
+
+// qk_compute_rho without allocating: the node charges accumulate in this
+// device's `scratch` (resolved by qk_create), cleared by a fill kernel first.
+// Calls that overlap on one GPU (two contexts on two threads) share it and race.
+int qk_compute_rho_noalloc(qk_ctx *c, double *rho)
+{
+    if (!c || !rho) return fail(QK_ERR_INVALID, "null argument");
+    if ((int64_t)c->g.ncell > kNoallocCells)
+        return fail(QK_ERR_CAPACITY, "qk_compute_rho_noalloc: the grid has " + std::to_string(c->g.ncell) +
+                                         " cells, above its limit of " + std::to_string(kNoallocCells) +
+                                         " (qk_compute_rho has none)");
+    double *d = scratch_ptr(c->p.device);
+    if (!d) return fail(QK_ERR_STATE, "qk_compute_rho_noalloc: no scratch resolved for this device");
+    const size_t n = (size_t)c->g.ncell * 3;
+    HIP_TRY(launch_zero(c->stream, d, n));
+    const int slots[3] = {QK_ELECTRON, QK_POSITRON, QK_PHOTON};
+    for (int t = 0; t < 3; ++t) {
+        SpeciesBuf &b = c->sp[slots[t]];
+        if (!b.active || b.n == 0) continue;
+        HIP_TRY(c->rsize == 4 ? launch_compute_rho<float>(c->stream, c->g, view<float>(b.buf[b.cur]), b.n, d, t)
+                              : launch_compute_rho<double>(c->stream, c->g, view<double>(b.buf[b.cur]), b.n, d, t));
+    }
+    HIP_TRY(hipMemcpyAsync(rho, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return QK_OK;
 }
 
