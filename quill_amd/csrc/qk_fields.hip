@@ -188,24 +188,30 @@ __global__ void __launch_bounds__(kBX * kBY) k_maxwell_e(Grid g, Coef<real> cf, 
 // the half that leaves the grid is dropped (4-point mean); cells on two or
 // more boundary planes keep their value.
 //
-// A thread owns one (j, k) column and marches along i over kBeRun planes,
-// carrying the 2x2 transverse sums of the previous plane in registers, so
-// each B value is read about once from L2 and four times from L1.
-constexpr int kBeRun = 8;   // 8 planes: 0.196 ms per step at 256x128x128, against 0.216 with 16
+// A thread owns one (j, k) column over kBeRun planes: it takes the 2x2
+// transverse sums of the kBeRun + 1 planes it needs first, with every load
+// unconditional (indices clamped into the grid, entries outside it dropped
+// by a select), so that all of them are in flight at once, then combines
+// neighbouring planes.  Marching plane by plane with the loads of one plane
+// at a time left each thread waiting on memory once per plane.
+constexpr int kBeRun = 8;
 
 // 2x2 sum of F in the (j, k) plane at i: rows j and j + dj, columns k and
-// k + dk, entries outside the grid left out; n counts the entries taken
+// k + dk, entries outside the grid left out (the same sum, in the same
+// order, as taking only the entries inside); n counts the entries taken
 template <typename real>
 __device__ inline real plane_sum(const Grid &g, const real *__restrict__ F, int i, int j, int k, int dj, int dk,
                                  int &n)
 {
     const int j2 = j + dj, k2 = k + dk;
     const bool vj = (unsigned)j2 < (unsigned)g.ny, vk = (unsigned)k2 < (unsigned)g.nz;
+    const int jc = vj ? j2 : j, kc = vk ? k2 : k;
     const real *r = F + i * g.sx;
-    real s = r[j * g.sy + k];
-    if (vk) s += r[j * g.sy + k2];
-    if (vj) s += r[j2 * g.sy + k];
-    if (vj && vk) s += r[j2 * g.sy + k2];
+    const real a = r[j * g.sy + k], b = r[j * g.sy + kc], c = r[jc * g.sy + k], d = r[jc * g.sy + kc];
+    real s = a;
+    if (vk) s += b;
+    if (vj) s += c;
+    if (vj && vk) s += d;
     n = (1 + vj) * (1 + vk);
     return s;
 }
@@ -218,29 +224,32 @@ __global__ void __launch_bounds__(kBX * kBY) k_interpolate_be(Grid g, const real
 {
     const int k = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
-    const int i0 = blockIdx.z * kBeRun, i1 = min(i0 + kBeRun, g.nx);
+    const int i0 = blockIdx.z * kBeRun;
     if (k >= g.nz || j >= g.ny) return;
     const int nbjk = (j == 0 || j == g.ny - 1) + (k == 0 || k == g.nz - 1);
-    // entries of each 2x2 transverse sum (the same on every plane)
-    int cx, cy, cz;
-    // x: planes i, i+1; y, z: planes i-1, i
-    real px = plane_sum(g, bx, i0, j, k, -1, -1, cx);
-    real qy = i0 > 0 ? plane_sum(g, by, i0 - 1, j, k, 1, -1, cy) : real(0);
-    real rz = i0 > 0 ? plane_sum(g, bz, i0 - 1, j, k, -1, 1, cz) : real(0);
-    for (int i = i0; i < i1; ++i) {
-        const bool up = i + 1 < g.nx;
-        const real px1 = up ? plane_sum(g, bx, i + 1, j, k, -1, -1, cx) : real(0);
-        const real qy1 = plane_sum(g, by, i, j, k, 1, -1, cy);
-        const real rz1 = plane_sum(g, bz, i, j, k, -1, 1, cz);
-        if (nbjk + (i == 0 || i == g.nx - 1) < 2) {
+    // sx[m]: Bx sum at plane i0 + m; sy[m], sz[m]: By, Bz sums at plane
+    // i0 - 1 + m (planes clamped into the grid; the clamped ones are unused)
+    real sx[kBeRun + 1], sy[kBeRun + 1], sz[kBeRun + 1];
+    int cx = 0, cy = 0, cz = 0;   // entries of each 2x2 sum (the same on every plane)
+#pragma unroll
+    for (int m = 0; m <= kBeRun; ++m) {
+        const int ix = min(i0 + m, g.nx - 1), iy = max(min(i0 - 1 + m, g.nx - 1), 0);
+        sx[m] = plane_sum(g, bx, ix, j, k, -1, -1, cx);
+        sy[m] = plane_sum(g, by, iy, j, k, 1, -1, cy);
+        sz[m] = plane_sum(g, bz, iy, j, k, -1, 1, cz);
+    }
+#pragma unroll
+    for (int m = 0; m < kBeRun; ++m) {
+        const int i = i0 + m;
+        if (i < g.nx && nbjk + (i == 0 || i == g.nx - 1) < 2) {
+            const bool up = i + 1 < g.nx;
+            const real px1 = up ? sx[m + 1] : real(0);
+            const real qy = i > 0 ? sy[m] : real(0), rz = i > 0 ? sz[m] : real(0);
             const int id = i * g.sx + j * g.sy + k;
-            bex[id] = (px + px1) * (cx * (1 + up) == 8 ? real(0.125) : real(0.25));
-            bey[id] = (qy + qy1) * (cy * (1 + (i > 0)) == 8 ? real(0.125) : real(0.25));
-            bez[id] = (rz + rz1) * (cz * (1 + (i > 0)) == 8 ? real(0.125) : real(0.25));
+            bex[id] = (sx[m] + px1) * (cx * (1 + up) == 8 ? real(0.125) : real(0.25));
+            bey[id] = (qy + sy[m + 1]) * (cy * (1 + (i > 0)) == 8 ? real(0.125) : real(0.25));
+            bez[id] = (rz + sz[m + 1]) * (cz * (1 + (i > 0)) == 8 ? real(0.125) : real(0.25));
         }
-        px = px1;
-        qy = qy1;
-        rz = rz1;
     }
 }
 
