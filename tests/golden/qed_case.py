@@ -125,6 +125,7 @@ def summary(n_ph, n_p, energy_ph, photon_g):
 
 
 NAMES = ["n_ph", "n_p", "energy_ph"] + [f"hist[{a:.3g},{b:.3g})" for a, b in zip(BINS[:-1], BINS[1:])]
+YIELDS_STATS = 3   # leading columns of summary that are yields
 
 
 def zscores(a, b):

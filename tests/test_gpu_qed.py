@@ -3,11 +3,12 @@
 Event sequences cannot match the reference (its get_rand is a sequential
 lagged-Fibonacci stream, src/mc.cpp:12-23; the GPU uses counter-based Philox
 streams), so parity is statistical: 16 seeded GPU runs against 16 runs of
-the reference's own code (tests/golden/make_golden.py).  Two cases: the
-chi ~ 1 beam of qed_stats.npz (|z| < 4 on every summary) and the pair-rich
-chi ~ 2 case of qed_pairs.npz (about 1400 pairs per run): yields within 3
-sigma (|z| < 3) and the photon and positron spectra each within 3 sigma as
-a whole (sum of z^2 over the bins, chi-square p > 0.0027).
+the reference's own code (tests/golden/make_golden.py).  Three cases: the
+chi ~ 1 beam of qed_stats.npz, the pair-rich chi ~ 2 case of qed_pairs.npz
+(about 1400 pairs per run) and the colliding-pulse cascade of
+qed_cascade.npz.  Each compares yields within 3 sigma (|z| < 3) and its
+spectra each within 3 sigma as a whole (sum of z^2 over the bins,
+chi-square p > 0.0027).
 """
 import os
 import sys
@@ -49,8 +50,12 @@ def test_qed_statistics_match_reference(precision):
     ref = gio.load("qed_stats")["summary"]
     mine = gpu_summaries(precision)
     z = qed_case.zscores(mine, ref)
-    bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES, z) if not abs(v) < 4]
-    assert not bad, f"GPU QED statistics differ from the reference: {bad}"
+    y = qed_case.YIELDS_STATS
+    print("z: " + ", ".join(f"{n} {v:.2f}" for n, v in zip(qed_case.NAMES, z)), flush=True)
+    bad = [(n, round(float(v), 2)) for n, v in zip(qed_case.NAMES[:y], z[:y]) if not abs(v) < 3]
+    assert not bad, f"GPU QED yields differ from the reference: {bad}"
+    p = qed_case.spectrum_chi2_p(z[y:])
+    assert p > 0.0027, (p, np.round(z[y:], 2))
 
 
 @pytest.mark.gpu
@@ -90,7 +95,8 @@ def test_vacuum_pair_rate():
     """birth_from_vacuum (src/mc.cpp:142-196): in a uniform E the number of
     pairs born in one step is binomial over the cells 2 .. n-3 with the
     per-cell probability W dt dV of the Schwinger rate (oracle rate, pinned
-    to the reference by rates.npz)."""
+    to the reference by rates.npz): the mean over 8 seeds within 3 standard
+    errors."""
     n = 24
     dx = 0.05 * 2 * np.pi
     dt = 0.025 * 2 * np.pi
@@ -112,7 +118,8 @@ def test_vacuum_pair_rate():
     assert (counts[:, 0] == counts[:, 1]).all()   # pairs
     mean, sd = cells * p, np.sqrt(cells * p * (1 - p))
     z = (counts[:, 1].mean() - mean) / (sd / np.sqrt(len(counts)))
-    assert mean > 20 and abs(z) < 4, (counts[:, 1], mean, z)
+    print(f"vacuum pairs: mean {counts[:, 1].mean():.1f} expected {mean:.1f}, z {z:.2f}", flush=True)
+    assert mean > 20 and abs(z) < 3, (counts[:, 1], mean, z)
 
 
 @pytest.mark.gpu
