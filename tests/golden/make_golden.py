@@ -12,6 +12,8 @@ Cases
              mixed particle population (electrons, positrons, ions, photons;
              QED off); outputs E, B, J, Be, particles (by id) and per-step
              diagnostics
+  run_edges  the same from zero fields with hand-placed particles on the
+             deposit's edge cases (tests/golden/edge_case.py)
   laser_cos  f_init_cos (src/init.cpp:6-210), sscos = 0, tilted and rotated
   rates      chi / w / tilde_w / mathcal_W (src/mc.cpp:26-116) on a table
   push       push_boris / push_vay (src/pusher.cpp) on a table
@@ -43,6 +45,7 @@ sys.path.insert(0, HERE)
 
 import golden_io as gio  # noqa: E402
 import film_case  # noqa: E402
+import edge_case  # noqa: E402
 import merge_case  # noqa: E402
 import mw_case  # noqa: E402
 import qed_case  # noqa: E402
@@ -112,6 +115,25 @@ def case_run(name, nx, ny, nz, dx, dy, dz, dt, solver, pusher, nsteps, seed, sig
     np.savez_compressed(os.path.join(HERE, name + ".npz"), header=h.tobytes(), E_in=E, B_in=B, part_in=rec,
                         ids_in=ids, E=Eo, B=Bo, J=Jo, Be=BEo, part=prec[order], ids=pids[order], diag=diag)
     print(f"{name}: {len(rec)} particles in, {len(prec)} out, {nsteps} steps")
+
+
+def case_edges():
+    """Hand-placed particles on the deposit's edge cases (edge_case.py): ties
+    between crossing fractions, starts on a face, ends within rounding of
+    one, exits through every side, three crossings, one crowded cell."""
+    nx, ny, nz = edge_case.SHAPE
+    rec, ids, icmr = edge_case.particles()
+    zero = np.zeros((nx, ny, nz, 3))
+    dx = edge_case.DX
+    h = gio.header(mode=gio.MODE_RUN, nx=nx, ny=ny, nz=nz, solver=oracle.FDTD, pusher=oracle.BORIS, qed=0,
+                   nsteps=edge_case.NSTEPS, n_particles=len(rec), seed=1, n_ion=1, dx=dx, dy=dx, dz=dx,
+                   dt=edge_case.DT, e_s=E_S, external_bz=0.0, **norms(dx, dx, dx))
+    h["ion_cmr"][0] = icmr
+    (Eo, Bo, Jo, BEo), prec, pids, diag = run_ref(h, zero, zero, rec, ids)
+    order = np.argsort(pids, kind="stable")
+    np.savez_compressed(os.path.join(HERE, "run_edges.npz"), header=h.tobytes(), E_in=zero, B_in=zero, part_in=rec,
+                        ids_in=ids, E=Eo, B=Bo, J=Jo, Be=BEo, part=prec[order], ids=pids[order], diag=diag)
+    print(f"run_edges: {len(rec)} particles in, {len(prec)} out, {edge_case.NSTEPS} steps")
 
 
 def case_laser():
@@ -390,6 +412,7 @@ def main(names):
                                          10, 303, 0.8, 0.5),
         "run_fdtd_vay_cold": lambda: case_run("run_fdtd_vay_cold", 12, 12, 12, dx, dx, dx, dt, oracle.FDTD,
                                               oracle.VAY, 12, 404, 0.01, 0.05),
+        "run_edges": case_edges,
         "laser_cos": case_laser,
         "rates": case_rates,
         "push": case_push,

@@ -55,7 +55,7 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / den if den > 0 else np.linalg.norm(a)
 
 
-@pytest.mark.parametrize("name", RUN_CASES)
+@pytest.mark.parametrize("name", RUN_CASES + ["run_edges"])
 def test_fp64_global_matches_reference(name):
     fx = gio.load(name)
     out = run_case(fx, "fp64", "global")
@@ -100,3 +100,28 @@ def test_fp32_matches_reference(name, deposit):
     for l, d in enumerate(out["diag"]):
         assert d["energy_f"] == pytest.approx(ref[l][6], rel=1e-5)
         assert d["n_e"] == pytest.approx(ref[l][3], rel=5e-3, abs=2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_deposit_edge_cases_tiled(precision):
+    """The hand-placed edge cases of run_edges.npz (ties between crossing
+    fractions, starts on a face, ends within rounding of one, exits through
+    every side, three crossings, eight particles in one cell; edge_case.py)
+    through the tiled kernel: FP64 to 1e-10 of the reference, FP32 per field
+    component within 1e-4 relative L2 and positions within 2e-4 cells."""
+    fx = gio.load("run_edges")
+    out = run_case(fx, precision, "tiled")
+    if precision == "fp64":
+        for key in ("E", "B", "J", "Be"):
+            err = np.abs(out[key] - fx[key]).max() / np.abs(fx[key]).max()
+            assert err < 1e-10, f"{key}: max rel err {err:.3e}"
+        np.testing.assert_array_equal(out["ids"], fx["ids"])
+        np.testing.assert_allclose(out["part"][:, :3], fx["part"][:, :3], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(out["part"][:, 3:6], fx["part"][:, 3:6], rtol=1e-9, atol=1e-12)
+        return
+    for key in ("E", "B", "J", "Be"):
+        for c in range(3):
+            err = rel_l2(out[key][..., c], fx[key][..., c])
+            assert err < 1e-4, f"{key}[{c}]: rel L2 {err:.3e}"
+    np.testing.assert_array_equal(out["ids"], fx["ids"])
+    np.testing.assert_allclose(out["part"][:, :3], fx["part"][:, :3], rtol=0, atol=2e-4)

@@ -20,7 +20,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 import golden_io as gio  # noqa: E402
 import oracle  # noqa: E402
 
-RUN_CASES = ["run_fdtd_boris", "run_ndfx_vay", "run_five_vay", "run_fdtd_vay_cold"]
+RUN_CASES = ["run_fdtd_boris", "run_ndfx_vay", "run_five_vay", "run_fdtd_vay_cold", "run_edges"]
 
 
 def hdr(fx, key="header"):
