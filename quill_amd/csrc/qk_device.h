@@ -206,31 +206,43 @@ __device__ inline void push(real cmr_dt, const real ef[3], const real bf[3], rea
 // ---------------------------------------------------------------------------
 // charge-conserving deposit of one straight segment (simple_jdep,
 // src/deposition.cpp:13-39).  p is the segment start relative to the cell
-// whose lower corner is w (so xa = p.x - w.x), d the segment.  The Acc policy
-// receives (component, di, dj, dk, value) with offsets relative to w.
-
+// whose lower corner is w, d the segment.  The Acc policy receives
+// (component, di, dj, dk, value) with offsets relative to w.
+//
+// The weights are the reference's, written at the segment's midpoint m = p +
+// d/2: for jx at the node (j + 1, k + 1) the reference's ya za + ya d2/2 +
+// za d1/2 + d1 d2/3 (ya = p.y, za = p.z) equals t = m.y m.z + d1 d2/12, and
+// the other three nodes of the square take m.z - t, m.y - t and
+// (1 - m.y - m.z) + t (the four sum to 1).  Same for jy over (x, z) and jz
+// over (x, y).  Fewer operations than the reference's form, equal up to
+// rounding.
 template <typename real, class Acc>
 __device__ inline void deposit_segment(Acc &acc, real q, real px, real py, real pz, real d0, real d1, real d2)
 {
-    const real xa = px, xb = real(1) - xa;
-    const real ya = py, yb = real(1) - ya;
-    const real za = pz, zb = real(1) - za;
-    const real third = real(1) / real(3);
+    const real h = real(0.5), c12 = real(1) / real(12);
+    const real xm = px + h * d0, ym = py + h * d1, zm = pz + h * d2;
     const real qx = d0 * q, qy = d1 * q, qz = d2 * q;
-    const real hyz = real(0.5) * d1, hzz = real(0.5) * d2, hxx = real(0.5) * d0;
-    const real yz3 = d1 * d2 * third, xz3 = d0 * d2 * third, xy3 = d0 * d1 * third;
-    acc(0, 0, 1, 1, qx * (ya * za + hzz * ya + hyz * za + yz3));
-    acc(0, 0, 0, 1, qx * (yb * za + hzz * yb - hyz * za - yz3));
-    acc(0, 0, 1, 0, qx * (ya * zb + hyz * zb - hzz * ya - yz3));
-    acc(0, 0, 0, 0, qx * (yb * zb - hzz * yb - hyz * zb + yz3));
-    acc(1, 1, 0, 1, qy * (xa * za + hxx * za + hzz * xa + xz3));
-    acc(1, 0, 0, 1, qy * (xb * za + hzz * xb - hxx * za - xz3));
-    acc(1, 1, 0, 0, qy * (xa * zb + hxx * zb - hzz * xa - xz3));
-    acc(1, 0, 0, 0, qy * (xb * zb - hxx * zb - hzz * xb + xz3));
-    acc(2, 1, 1, 0, qz * (xa * ya + hxx * ya + hyz * xa + xy3));
-    acc(2, 0, 1, 0, qz * (xb * ya + hyz * xb - hxx * ya - xy3));
-    acc(2, 1, 0, 0, qz * (xa * yb + hxx * yb - hyz * xa - xy3));
-    acc(2, 0, 0, 0, qz * (xb * yb - hxx * yb - hyz * xb + xy3));
+    {
+        const real t = ym * zm + d1 * d2 * c12;
+        acc(0, 0, 1, 1, qx * t);
+        acc(0, 0, 0, 1, qx * (zm - t));
+        acc(0, 0, 1, 0, qx * (ym - t));
+        acc(0, 0, 0, 0, qx * ((real(1) - ym - zm) + t));
+    }
+    {
+        const real t = xm * zm + d0 * d2 * c12;
+        acc(1, 1, 0, 1, qy * t);
+        acc(1, 0, 0, 1, qy * (zm - t));
+        acc(1, 1, 0, 0, qy * (xm - t));
+        acc(1, 0, 0, 0, qy * ((real(1) - xm - zm) + t));
+    }
+    {
+        const real t = xm * ym + d0 * d1 * c12;
+        acc(2, 1, 1, 0, qz * t);
+        acc(2, 0, 1, 0, qz * (ym - t));
+        acc(2, 1, 0, 0, qz * (xm - t));
+        acc(2, 0, 0, 0, qz * ((real(1) - xm - ym) + t));
+    }
 }
 
 // Split the straight path d, starting at p (relative to the start cell, so
