@@ -251,6 +251,15 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
     Corners<real> nodes1;
     if (p < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
 #endif
+    // Each round stores the record of the round before it, right after
+    // issuing its own loads: gfx9 counts loads and stores in one vmcnt, so
+    // the loop's wait for those loads (one round later) then waits for
+    // stores that have had a whole push to complete, not for stores just
+    // issued.  The first round rewrites its own record unchanged, and free
+    // slots are rewritten unchanged, which keeps the stores unconditional.
+    PRec<real> rs = r1;
+    uint32_t ps = p;
+    const bool started = p < end32;
     for (; p < end32; p += kTBlock) {
         PRec<real> r = r1;
 #if QK_GATHER_AHEAD
@@ -266,11 +275,13 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
             if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
 #endif
             if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, r2);
+            store_rec32(s, ps, rs);
         } else {
             if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r1);
 #if QK_GATHER_AHEAD
             if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
 #endif
+            store_rec32(s, ps, rs);
         }
         if (r.cell >= 0) {
 #if !QK_GATHER_AHEAD
@@ -278,9 +289,11 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
             gather(r, ri, rj, rk, in_win, nodes);
 #endif
             advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
-            store_rec32(s, p, r);
         }
+        rs = r;
+        ps = p;
     }
+    if (started) store_rec32(s, ps, rs);
     __syncthreads();
 
     {
