@@ -190,16 +190,28 @@ __global__ void __launch_bounds__(kSortBlock) k_count_keep(const int *__restrict
     __syncthreads();
     const int s = old_start[t], e = old_start[t + 1];
     int kept = 0;
-    for (int p = s + threadIdx.x; p < e; p += kSortBlock) {
-        const int c = cell[p];
-        if (c < 0) continue;
-        const int tn = tile_of_cell(tg, c);
-        if (tn == t) {
-            ++kept;
-        } else {
-            const int b = neighbour_bin(tg, t, tn);
-            if (b >= 0) atomicAdd(&bins[b], 1);
-            else atomicAdd(imm + tn, 1);
+    // kCountIt cells per thread and pass, all loaded before the first is
+    // used, so that a thread waits for memory once per pass
+    constexpr int kCountIt = 4;
+    for (int p0 = s + threadIdx.x; p0 < e; p0 += kCountIt * kSortBlock) {
+        int cs[kCountIt];
+#pragma unroll
+        for (int m = 0; m < kCountIt; ++m) {
+            const int p = p0 + m * kSortBlock;
+            cs[m] = p < e ? cell[p] : -1;
+        }
+#pragma unroll
+        for (int m = 0; m < kCountIt; ++m) {
+            const int c = cs[m];
+            if (c < 0) continue;
+            const int tn = tile_of_cell(tg, c);
+            if (tn == t) {
+                ++kept;
+            } else {
+                const int b = neighbour_bin(tg, t, tn);
+                if (b >= 0) atomicAdd(&bins[b], 1);
+                else atomicAdd(imm + tn, 1);
+            }
         }
     }
     for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
