@@ -40,15 +40,9 @@ constexpr int kT[3] = {kTx, kTy, kTz};
 #define QK_TATTR
 #endif
 constexpr int kTBlock = QK_TBLOCK;
-// software pipeline of the tiled kernel: gather the next round's field nodes
-// while the current particle is pushed (1) or at use (0); records prefetched
-// one or two rounds ahead
-#ifndef QK_GATHER_AHEAD
-#define QK_GATHER_AHEAD 1
-#endif
-#ifndef QK_RECORDS_AHEAD
-#define QK_RECORDS_AHEAD 2
-#endif
+// out-of-window particles listed per workgroup and pushed after its rounds:
+// at most this many per chunk, the rest inline
+constexpr int kDefer = 512;
 
 // Current window, FP64, one block per component.  A deposit from start cell
 // r touches cells r-1 .. r+1 (one face crossing per axis at most) and, for
@@ -205,6 +199,13 @@ __global__ void __launch_bounds__(kBlock) k_push_global(Grid g, Fields<real> f, 
 }
 
 
+// A particle that starts outside the window is not pushed in its round but
+// listed (its slot) in LDS, and the workgroup pushes the listed particles
+// after its rounds, one per thread, through global memory.  Inline, every
+// wavefront with one such lane would run the global gather and deposit for
+// it; listed, they run densely (the push measured 3.8 % faster at
+// 512x256x256, where they gather in the laser's wake, and 1.5 % slower at
+// 256x128x128, where a few per chunk make a short serial tail).
 template <typename real, int PUSHER>
 __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<real> f, Current<real> cur, Species<real> s,
                                                        StepConst<real> sc, const int4 *__restrict__ chunks,
@@ -212,6 +213,8 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
 {
     __shared__ double win[kJWin];
     __shared__ real stage[2 * kFStage];
+    __shared__ uint32_t listed[kDefer];
+    __shared__ int nlisted;
     // chunks in launch order: the round-robin deal of workgroups to the XCDs
     // spreads neighbouring tiles over all eight, which measured faster here
     // than grouping them on one XCD (unlike the scatter, see qk_sort.hip)
@@ -231,8 +234,9 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
     uint32_t p = (uint32_t)ch.y + threadIdx.x;
     PRec<real> r1, r2;   // the records one and two rounds ahead
     if (p < end32) load_rec32(s, p, r1);
-    if (QK_RECORDS_AHEAD > 1 && p + kTBlock < end32) load_rec32(s, p + kTBlock, r2);
+    if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r2);
     for (int w = threadIdx.x; w < kJWin; w += kTBlock) win[w] = 0.0;
+    if (threadIdx.x == 0) nlisted = 0;
     {
         const int si = i0 - kHalo, sj = j0 - kHalo, sk = k0 - kHalo;
         real v[6][kStageIt];
@@ -270,67 +274,81 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
     // software pipeline over the rounds of kTBlock particles: while one
     // particle is pushed, the next round's record has its field nodes
     // gathered and the record two rounds ahead is loading (the first two
-    // rounds' records were issued before the staging)
-    auto gather = [&](const PRec<real> &q, int &a, int &b, int &cc, bool &w, Corners<real> &nd) {
+    // rounds' records were issued before the staging).  listed_q: the
+    // particle was listed for the pass after the rounds.
+    auto gather = [&](const PRec<real> &q, uint32_t slot, int &a, int &b, int &cc, bool &w, bool &listed_q,
+                      Corners<real> &nd) {
+        listed_q = false;
         if (q.cell < 0) return;
         a = cell_i(q.cell) - i0;
         b = cell_j(q.cell) - j0;
         cc = cell_k(q.cell) - k0;
         w = (unsigned)(a + kHalo) < (unsigned)(kTx + 2 * kHalo) && (unsigned)(b + kHalo) < (unsigned)(kTy + 2 * kHalo) &&
             (unsigned)(cc + kHalo) < (unsigned)(kTz + 2 * kHalo);
-        if (w) load_corners_stage(stage, a, b, cc, nd);
-        else load_corners(f, cell_base(g, q.cell), g.sx, g.sy, nd);
+        if (w) {
+            load_corners_stage(stage, a, b, cc, nd);
+            return;
+        }
+        const int at = atomicAdd(&nlisted, 1);
+        if (at < kDefer) {
+            listed[at] = slot;
+            listed_q = true;
+            return;
+        }
+        load_corners(f, cell_base(g, q.cell), g.sx, g.sy, nd);
     };
-#if QK_GATHER_AHEAD
     int ri1 = 0, rj1 = 0, rk1 = 0;
-    bool w1 = false;
+    bool w1 = false, l1 = false;
     Corners<real> nodes1;
-    if (p < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
-#endif
+    if (p < end32) gather(r1, p, ri1, rj1, rk1, w1, l1, nodes1);
     // Each round stores the record of the round before it, right after
     // issuing its own loads: gfx9 counts loads and stores in one vmcnt, so
     // the loop's wait for those loads (one round later) then waits for
     // stores that have had a whole push to complete, not for stores just
     // issued.  The first round rewrites its own record unchanged, and free
-    // slots are rewritten unchanged, which keeps the stores unconditional.
+    // and listed slots are rewritten unchanged, which keeps the stores
+    // unconditional.
     PRec<real> rs = r1;
     uint32_t ps = p;
     const bool started = p < end32;
     for (; p < end32; p += kTBlock) {
         PRec<real> r = r1;
-#if QK_GATHER_AHEAD
         Corners<real> nodes = nodes1;
         ri = ri1;
         rj = rj1;
         rk = rk1;
         in_win = w1;
-#endif
-        if (QK_RECORDS_AHEAD > 1) {
-            r1 = r2;
-#if QK_GATHER_AHEAD
-            if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
-#endif
-            if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, r2);
-            store_rec32(s, ps, rs);
-        } else {
-            if (p + kTBlock < end32) load_rec32(s, p + kTBlock, r1);
-#if QK_GATHER_AHEAD
-            if (p + kTBlock < end32) gather(r1, ri1, rj1, rk1, w1, nodes1);
-#endif
-            store_rec32(s, ps, rs);
-        }
-        if (r.cell >= 0) {
-#if !QK_GATHER_AHEAD
-            Corners<real> nodes;
-            gather(r, ri, rj, rk, in_win, nodes);
-#endif
-            advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
-        }
+        const bool is_listed = l1;
+        r1 = r2;
+        if (p + kTBlock < end32) gather(r1, p + kTBlock, ri1, rj1, rk1, w1, l1, nodes1);
+        if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, r2);
+        store_rec32(s, ps, rs);
+        if (r.cell >= 0 && !is_listed) advance_particle<real, PUSHER>(g, nodes, r, sc, dep, da);
         rs = r;
         ps = p;
     }
     if (started) store_rec32(s, ps, rs);
+    // the listed slots' rewrites complete before the pass below stores them
+    // again (vmcnt 0; expcnt and lgkmcnt left at their maxima)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
+    {
+        auto gdep = [&](int, int, int, int base, real q, real pos[3], const real d[3], DiagAcc &acc_d) {
+            GlobalAcc<real> acc{cur.jx, cur.jy, cur.jz, g.sx, g.sy, base, base};
+            deposit_path<real>(acc, q, pos, d);
+            acc_d.fallback += 1;
+        };
+        const int nl = min(nlisted, kDefer);
+        for (int e = threadIdx.x; e < nl; e += kTBlock) {
+            const uint32_t q = listed[e];
+            PRec<real> rr;
+            load_rec32(s, q, rr);
+            Corners<real> nn;
+            load_corners(f, cell_base(g, rr.cell), g.sx, g.sy, nn);
+            advance_particle<real, PUSHER>(g, nn, rr, sc, gdep, da);
+            store_rec32(s, q, rr);
+        }
+    }
 
     {
         const int wi0 = i0 - kHalo - 1, wj0 = j0 - kHalo - 1, wk0 = k0 - kHalo - 1;
