@@ -192,17 +192,17 @@ __device__ inline double kinetic(const real u[3], real gam)
     return (double)((u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * qk_rcp(gam + real(1)));
 }
 
-// One charged particle with QED off, given the field nodes around it (loaded
-// by load_corners): interpolate E and Be (e_to_particle / b_to_particle,
-// src/deposition.cpp:431-471), add external Bz, push (Vay or Boris), deposit
+// One charged particle with QED off, given E and Be interpolated to it
+// (interp_corners of the field nodes around it: e_to_particle /
+// b_to_particle, src/deposition.cpp:431-471), add external Bz, push (Vay or Boris), deposit
 // the current of the straight path u dt / gamma split at cell faces, move,
 // and mark the slot free if the new cell is not is_inside (place(),
 // src/plist.cpp:7-40).  Diagnostics of compute_N / compute_energy
 // (src/fout.cpp:41-138) come from the post-push state.  dep(i, j, k, base,
 // q, pos, d, da) performs the deposit and advances pos.
 template <typename real, int PUSHER, class Dep>
-__device__ inline void advance_particle(const Grid &g, const Corners<real> &nodes, PRec<real> &r,
-                                        const StepConst<real> &sc, Dep &&dep, DiagAcc &da)
+__device__ inline void advance_particle_eb(const Grid &g, const real (&ei)[3], const real (&bi)[3], PRec<real> &r,
+                                           const StepConst<real> &sc, Dep &&dep, DiagAcc &da)
 {
     const int c = r.cell;
     if (c < 0) return;
@@ -218,8 +218,7 @@ __device__ inline void advance_particle(const Grid &g, const Corners<real> &node
     real pos[3] = {r.o[0], r.o[1], r.o[2]};
     real u[3] = {r.u[0], r.u[1], r.u[2]};
     const int base = i * g.sx + j * g.sy + k;
-    real e[3], b[3];
-    interp_corners(nodes, pos[0], pos[1], pos[2], e, b);
+    real e[3] = {ei[0], ei[1], ei[2]}, b[3] = {bi[0], bi[1], bi[2]};
     b[2] += sc.ext_bz;
     real gam = qk_sqrt(real(1) + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     push<PUSHER>(sc.cmr_dt, e, b, u, gam);
@@ -250,6 +249,17 @@ __device__ inline void advance_particle(const Grid &g, const Corners<real> &node
     } else {
         r.cell = -1;
     }
+}
+
+// the same from the field nodes around the particle (load_corners)
+template <typename real, int PUSHER, class Dep>
+__device__ inline void advance_particle(const Grid &g, const Corners<real> &nodes, PRec<real> &r,
+                                        const StepConst<real> &sc, Dep &&dep, DiagAcc &da)
+{
+    if (r.cell < 0) return;
+    real e[3], b[3];
+    interp_corners(nodes, r.o[0], r.o[1], r.o[2], e, b);
+    advance_particle_eb<real, PUSHER>(g, e, b, r, sc, dep, da);
 }
 
 }  // namespace
