@@ -331,9 +331,17 @@ __global__ void __launch_bounds__(kTBlock) QK_TATTR k_push_tiled(Grid g, Fields<
         r1 = r2;
         const bool more = p + kTBlock < end32;
         Corners<real> nd;
+        // The round's memory operations (the next particle's LDS reads, the
+        // record two rounds ahead, the previous record's store) are issued
+        // at a raised wavefront priority, so that they reach the memory
+        // system ahead of the other wavefronts' arithmetic: push 1.3 %
+        // faster at both 512x256x256 and 256x128x128 (priority 1 or 3, or
+        // raised also for the staging or the flush, measured less).
+        __builtin_amdgcn_s_setprio(2);
         if (more) gather(r1, p + kTBlock, ri1, rj1, rk1, w1, l1, nd);
         if (p + 2 * kTBlock < end32) load_rec32(s, p + 2 * kTBlock, r2);
         store_rec32(s, ps, rs);
+        __builtin_amdgcn_s_setprio(0);
         if (r.cell >= 0 && !is_listed) advance_particle_eb<real, PUSHER>(g, e, b, r, sc, dep, da);
         if (more && r1.cell >= 0 && !l1) interp_corners(nd, r1.o[0], r1.o[1], r1.o[2], e1, b1);
         rs = r;
